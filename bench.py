@@ -1,0 +1,196 @@
+"""Benchmark: PFDD GL steps/s on the 128^3 x 12-slip-system workload (c1) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-mixed]
+
+A step is one GL iteration (update, forward 3D FFT, B(k) contraction, inverse
+3D FFT) over the whole 128^3 x 12 instance, inputs resident in HBM.  Prints one
+JSON line.  value = FP64 GL steps/s; the mixed-precision path (FP32 FFT and
+contraction, FP64 master xi) is timed beside it with its error against FP64.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import pfdd_amd as pa  # noqa: E402
+from pfdd_amd import configs as C  # noqa: E402
+from pfdd_amd import hip  # noqa: E402
+
+METRIC = "GL steps/s at 128^3×12 slip, FP64 vs mixed FP32 at stated error; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+STAGES = [("k_update", pa.STAGE_UPDATE, 1), ("k_fft_rows(N3)", pa.STAGE_FFT_AXIS3, 2),
+          ("k_fft_cols(N2)", pa.STAGE_FFT_AXIS2, 2), ("k_fft_cols(N1)", pa.STAGE_FFT_AXIS1, 2),
+          ("k_contract", pa.STAGE_CONTRACT, 1)]
+
+
+def stage_bytes(stage, S, P, ntri, fp64, stored):
+    """Algorithmic bytes of one launch: each input read once, each output written once."""
+    c = 16 if fp64 else 8
+    r = 8 if fp64 else 4
+    if stage == pa.STAGE_UPDATE:
+        return S * P * (c + 16 + 16 + c)          # read W, read xi, write xi, write W
+    if stage in (pa.STAGE_FFT_AXIS3, pa.STAGE_FFT_AXIS2, pa.STAGE_FFT_AXIS1):
+        return S * P * 2 * c
+    if stage == pa.STAGE_CONTRACT:
+        return S * P * 2 * c + (ntri * P * r if stored else 0)
+    if stage == pa.STAGE_COPY:
+        return S * P * (16 + c)
+    raise ValueError(stage)
+
+
+def step_alg_bytes(S, P, fp64, stored):
+    """Stage-minimum bytes of one GL step (C2C, FP64: (144 S [+4 S(S+1)]) P;
+    mixed: (96 S [+2 S(S+1)]) P)."""
+    if fp64:
+        return (144 * S + (4 * S * (S + 1) if stored else 0)) * P
+    return (96 * S + (2 * S * (S + 1) if stored else 0)) * P
+
+
+def timed_steps(ctx, steps, warmup):
+    ctx.gl_steps(warmup, record=False)
+    ctx.sync()
+    e0, e1 = hip.Event(), hip.Event()
+    ctx.sync()
+    t0 = time.perf_counter()
+    e0.record(ctx.stream)
+    ctx.gl_steps(steps, record=False)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_ms(e1), wall
+
+
+def time_stages(ctx, reps=20):
+    out = {}
+    for name, st, per_step in STAGES + [("k_xi_to_w(copy)", pa.STAGE_COPY, 0)]:
+        ctx.enqueue_stage(st, 2)
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(ctx.stream)
+        ctx.enqueue_stage(st, reps)
+        e1.record(ctx.stream)
+        e1.synchronize()
+        out[name] = (st, per_step, e0.elapsed_ms(e1) / reps)
+    return out
+
+
+def make_ctx(w, precision, bmode):
+    ctx = pa.Context(w.params(precision, bmode))
+    ctx.set_tau(w.tau)
+    ctx.upload_xi(w.xi)
+    ctx.begin_outer(True)
+    return ctx
+
+
+def cpu_baseline(w, budget_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oraclepy as O
+    cores = max(1, min(16, os.cpu_count() or 1))
+    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * w.ns, C.asf2_reference(w.ns), [1.0] * w.ns,
+                      nthreads=cores)
+    o = O.Oracle(p)
+    o.set_xi(w.xi)
+    o.set_tau(w.tau)
+    o.prologue(True)
+    t = time.perf_counter()
+    o.gl_steps(1)
+    t1 = time.perf_counter() - t
+    n = int(max(1, min(100, budget_s / max(t1, 1e-6))))
+    t = time.perf_counter()
+    o.gl_steps(n)
+    dt = time.perf_counter() - t
+    o.close()
+    return {"value": n / dt, "unit": "GL steps/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle (oracle/pfdd_oracle.c, OpenMP, own Stockham FFT) on the same 128^3x12 c1 "
+                      f"instance: {n} GL steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--bmode", choices=["recompute", "stored"], default="recompute")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: the PFDD GL step is a single-GPU path here "
+              "(the 3D-FFT all-to-all sharding is not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if hip.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    bmode = pa.B_RECOMPUTE if a.bmode == "recompute" else pa.B_STORED
+    stored = bmode == pa.B_STORED
+    w = C.c1(bmode)
+    S, P = w.ns, w.P
+    ntri = S * (S + 1) // 2
+
+    # ---- FP64 (value)
+    ctx = make_ctx(w, pa.FP64, bmode)
+    ms64, wall64 = timed_steps(ctx, a.steps, a.warmup)
+    xi64 = ctx.download_xi()
+    stages = time_stages(ctx)
+    ctx.close()
+    sps64 = a.steps / (ms64 / 1e3)
+
+    # dominant kernel of the FP64 step and its roofline
+    dom = max(((n, st, k, ms) for n, (st, k, ms) in stages.items() if k > 0), key=lambda t: t[2] * t[3])
+    dname, dst, _, dms = dom
+    dbytes = stage_bytes(dst, S, P, ntri, True, stored)
+    achieved = dbytes / (dms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dname,
+                "alg_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5)}
+    step_bytes = step_alg_bytes(S, P, True, stored)
+    step_gbs = step_bytes * sps64 / 1e9
+    kernels = {n: {"avg_ms": round(ms, 5), "launches_per_step": k,
+                   "GB_s": round(stage_bytes(st, S, P, ntri, True, stored) / (ms / 1e3) / 1e9, 1)}
+               for n, (st, k, ms) in stages.items()}
+
+    out = {"metric": METRIC, "value": round(sps64, 2), "unit": "GL steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(ms64 / a.steps, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: seeded loops (SplitMix64, seed 20261202+1), sigma_ij ~ U[-0.02,0.02]",
+           "config": {"workload": "c1: 128^3 grid x 12 FCC slip systems, 16 seeded circular loops r~U[5,20], "
+                                  "FP64 GL step (FFT + B(k) contraction + update)",
+                      "grid": [128, 128, 128], "ns": S, "precision": "fp64", "bmode": a.bmode,
+                      "parallelism": "single GPU"},
+           "roofline": roofline,
+           "step_roofline": {"alg_bytes_per_step": step_bytes, "achieved_GB_s": round(step_gbs, 1),
+                             "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+           "gridpoints_x_systems_per_s": round(P * S * sps64, 1),
+           "kernels_fp64": kernels, "wall_s_fp64": round(wall64, 3)}
+
+    # ---- mixed precision
+    if not a.no_mixed:
+        ctxm = make_ctx(w, pa.MIXED, bmode)
+        ms32, _ = timed_steps(ctxm, a.steps, a.warmup)
+        xi32 = ctxm.download_xi()
+        ctxm.close()
+        sps32 = a.steps / (ms32 / 1e3)
+        err_xi = float(np.abs(xi32 - xi64).max())
+        area64 = float(xi64.real.sum())
+        area32 = float(xi32.real.sum())
+        out["mixed"] = {"value": round(sps32, 2), "unit": "GL steps/s", "speedup_vs_fp64": round(sps32 / sps64, 3),
+                        "xi_max_abs_err_vs_fp64": err_xi,
+                        "slipped_area_rel_err_vs_fp64": abs(area32 - area64) / abs(area64),
+                        "after_steps": a.steps + a.warmup,
+                        "step_roofline_frac": round(step_alg_bytes(S, P, False, stored) * sps32 / 1e9 / HBM_PEAK_GBS, 4)}
+        del xi32
+    del xi64
+
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * pfdd_gpu.h -- C ABI of libpfdd_gpu.so: the PFDD FFT-elasticity +
+ * Ginzburg-Landau step on one MI355X (gfx950), hand-written HIP kernels.
+ *
+ * The reference (PFDD_Final_GPU.c, "G.c") has no library boundary; its two
+ * seams are the FFT dispatch (fft_forward/fft_backward, G.c:209-273) and the
+ * GL loop body inside main (G.c:4574-4691).  Each entry point below names the
+ * reference code it replaces.
+ *
+ * Conventions
+ *  - Return codes: 0 on success, a negative PFDD_E_* code otherwise; nothing
+ *    asserts or exits inside the library.  pfdd_strerror() names the code.
+ *  - Host arrays use the reference layouts: complex fields are interleaved
+ *    doubles [field][N1][N2][N3][2] (G.c:38-43), N3 fastest.
+ *  - One context owns its device memory and one HIP stream.  Calls on a
+ *    context are asynchronous on that stream unless they return host data.
+ *    Contexts share no state; use one context per host thread.
+ *  - Only the NS slip-system fields evolve.  The reference's NV=9 virtual
+ *    strain fields are identically zero on the GL path (xi_v = 0, xi_bc = 1);
+ *    pfdd_upload_xi rejects non-zero virtual fields with PFDD_E_UNSUPPORTED.
+ */
+#ifndef PFDD_GPU_H
+#define PFDD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFDD_MAX_NS 16
+
+enum {
+  PFDD_OK = 0,
+  PFDD_E_HIP = -1,          /* a HIP runtime call failed */
+  PFDD_E_ALLOC = -2,        /* device or host allocation failed */
+  PFDD_E_SHAPE = -3,        /* bad grid / NS / batch / pointer argument */
+  PFDD_E_RADIX = -4,        /* a grid edge is not a supported FFT length */
+  PFDD_E_NAN = -5,          /* non-finite value in xi or in a parameter */
+  PFDD_E_UNSUPPORTED = -6,  /* input outside what the GL path supports */
+  PFDD_E_STATE = -7         /* call order (e.g. steps before upload) */
+};
+
+enum { PFDD_FP64 = 0, PFDD_MIXED = 1 };          /* precision of FFT + contraction */
+enum { PFDD_B_STORED = 0, PFDD_B_RECOMPUTE = 1 }; /* interaction matrix B(k) */
+enum { PFDD_FORWARD = -1, PFDD_BACKWARD = 1 };    /* FFT sign (G.c:106, FFTW_FORWARD) */
+
+/* Runtime replacement for the compile-time constants of G.c:81-95 and the
+ * constants set in main (G.c:4186-4300). */
+typedef struct {
+  int n1, n2, n3;              /* grid; each edge 2^a 3^b, 1 < n <= 1024 */
+  int ns;                      /* active slip systems, 1..PFDD_MAX_NS */
+  int precision;               /* PFDD_FP64 | PFDD_MIXED */
+  int bmode;                   /* PFDD_B_STORED | PFDD_B_RECOMPUTE */
+  double C11, C12, C44;        /* elastic constants; mu = C44 (G.c:337) */
+  double d1, d2, d3;           /* grid spacing of frec (G.c:4266-4268) */
+  double cd2[PFDD_MAX_NS];     /* CD2: dt * kinetic coefficient (G.c:4290) */
+  double asf2[PFDD_MAX_NS];    /* Asf2: gamma-surface amplitude (G.c:4278-4285) */
+  double dslip[PFDD_MAX_NS];   /* dslip2 (setMat G.c:2480) */
+  double uq2;                  /* 1.0 (G.c:4195) */
+  double pi_const;             /* 3.141592654 (G.c:95) */
+  double eps[PFDD_MAX_NS][3][3]; /* eigenstrains e_a (seteps G.c:1912-1980) */
+} pfdd_params;
+
+typedef struct pfdd_ctx pfdd_ctx;
+
+/* Allocate device state and build what B(k) needs (replaces the setup calls
+ * frec + Bmatrix of main, G.c:4322-4325; B is built on the device in FP64 and
+ * stored in the working precision when bmode == PFDD_B_STORED). */
+int pfdd_create(const pfdd_params *p, pfdd_ctx **out);
+void pfdd_destroy(pfdd_ctx *c);
+
+/* Resolved shear tau_a (resolSS G.c:1607-1622): tau[ns] when per_point == 0,
+ * or the reference array tau[N1][N2][N3][NS] when per_point == 1. */
+int pfdd_set_tau(pfdd_ctx *c, const double *tau, int per_point);
+
+/* Copy the order parameter in.  xi and xi_bc are the reference arrays
+ * [nsv][N1][N2][N3][2] with nsv >= ns (G.c:4146-4147); fields ns..nsv-1 are
+ * the virtual strains and must be zero.  Only the real slot of xi_bc is read
+ * (G.c:3480); xi_bc may be NULL (everything evolves). */
+int pfdd_upload_xi(pfdd_ctx *c, const double *xi, const double *xi_bc, int nsv);
+
+/* Copy the order parameter out into xi[nsv][N1][N2][N3][2]; virtual fields
+ * (ns..nsv-1) are written as zero.  Synchronous. */
+int pfdd_download_xi(pfdd_ctx *c, double *xi, int nsv);
+
+/* Outer-step prologue: data = FFT(xi); data2 = B.data, plus, when quirk != 0,
+ * the term data_b * sum_v DD_b,v that virtualevolv adds before its inverse
+ * transform (G.c:4450-4452, G.c:1426-1447); data2 = IFFT(data2).  The first
+ * GL step after it reads that data2. */
+int pfdd_begin_outer(pfdd_ctx *c, int quirk);
+
+/* nsteps iterations of the GL loop body in fixed-step mode (G.c:4574-4644
+ * with interfacevolv off): plasticevolv, FFT(xi), contraction with B(k),
+ * inverse FFT.  gamma[nsteps] and gamma_a[nsteps][ns] (may be NULL) receive
+ * the sums of plasticevolv (G.c:3492-3493).  Synchronous when gamma or
+ * gamma_a is non-NULL, asynchronous otherwise. */
+int pfdd_gl_steps(pfdd_ctx *c, int nsteps, double *gamma, double *gamma_a);
+
+/* Strain and stress of the current xi (strain G.c:752-835, stress
+ * G.c:837-972, averages G.c:4058-4070).  avepst_uniform[3][3] is the part
+ * of avepst that does not depend on xi, S:sigma_applied*mu (avestrain
+ * G.c:714-729); the xi-dependent part sum_a e_a <xi_a> is added on the
+ * device.  Fills aveps[3][3] and avesigma[3][3] (not divided by mu);
+ * eps_field / sig_field ([3][3][N1][N2][N3] doubles, real parts) may be
+ * NULL.  Synchronous.  It reuses the FFT work array, so (as in main, where
+ * calc_strain_stress precedes the GL loop, G.c:4461) pfdd_begin_outer must
+ * run again before the next pfdd_gl_steps (PFDD_E_STATE otherwise). */
+int pfdd_strain_stress(pfdd_ctx *c, const double *avepst_uniform, double *aveps,
+                       double *avesigma, double *eps_field, double *sig_field);
+
+/* Raw FFT seam (fft_forward / fft_backward, G.c:209-273): batched, in place,
+ * unnormalised 3D C2C on a device buffer of batch contiguous N1*N2*N3 complex
+ * blocks.  dev_buf is double2 when prec == PFDD_FP64, float2 otherwise.
+ * dir = PFDD_FORWARD (-1) or PFDD_BACKWARD (+1).  Asynchronous. */
+int pfdd_fft(pfdd_ctx *c, void *dev_buf, int batch, int dir, int prec);
+
+/* Wait for the context's stream. */
+int pfdd_sync(pfdd_ctx *c);
+
+/* The context's hipStream_t (as void*), for timing with HIP events. */
+void *pfdd_stream(pfdd_ctx *c);
+
+/* Device bytes held by the context. */
+long long pfdd_device_bytes(const pfdd_ctx *c);
+
+/* Profiling aid, no reference counterpart: enqueue `reps` launches of one
+ * kernel stage of the GL step on the context stream, so that HIP events or
+ * rocprofv3 can time it in isolation.  It overwrites the xi / work-array
+ * state (upload xi again afterwards).  Asynchronous. */
+enum {
+  PFDD_STAGE_UPDATE = 0,    /* plasticevolv + copy + gamma sums */
+  PFDD_STAGE_FFT_AXIS3 = 1, /* one pass along N3 (contiguous) over all fields */
+  PFDD_STAGE_FFT_AXIS2 = 2, /* one pass along N2 */
+  PFDD_STAGE_FFT_AXIS1 = 3, /* one pass along N1 */
+  PFDD_STAGE_CONTRACT = 4,  /* B(k) contraction */
+  PFDD_STAGE_STEP = 5,      /* one whole GL step */
+  PFDD_STAGE_COPY = 6       /* plain copy xi -> work array (bandwidth reference) */
+};
+int pfdd_enqueue_stage(pfdd_ctx *c, int stage, int reps);
+
+const char *pfdd_strerror(int rc);
+
+/* ---- host-side setup helpers (pure host code) ---------------------------
+ * These restate the reference's setup routines with runtime sizes so that a
+ * driver can build inputs without its compile-time constants. */
+
+/* The 12 FCC slip systems of setfcc (G.c:1812-1909): n = {111}/sqrt3 with a
+ * float sqrt3, b = <110>/2. */
+void pfdd_setfcc(double xn[12][3], double xb[12][3]);
+
+/* Orientation rows as setorient normalises them (G.c:2036-2055: each row
+ * divided by its squared length). */
+void pfdd_setorient(const double rows[3][3], double theta[3][3]);
+
+/* e_a = theta (b_a x n_a / dslip_a) theta^T, unsymmetrised (seteps G.c:1950-1978). */
+void pfdd_seteps(int ns, const double (*xn)[3], const double (*xb)[3], const double *dslip,
+                 const double theta[3][3], double (*eps)[3][3]);
+
+/* tau_a = sum_ij sigma_ij e_a,ij for a uniform sigma (resolSS G.c:1607-1622). */
+void pfdd_resolve_tau(int ns, const double sigma[3][3], const double (*eps)[3][3], double *tau);
+
+/* SplitMix64 stream (Steele, Lea & Flood 2014) and a uniform double in [0,1). */
+unsigned long long pfdd_splitmix64(unsigned long long *state);
+double pfdd_uniform01(unsigned long long *state);
+
+/* Voxelised circular loop: sets xi[sys] = 1 (real part) at the grid points x
+ * with |n.(x-c)| < h/2 and |(x-c) - (n.(x-c)) n| <= r, where x-c is the
+ * periodic minimum image, n = normal/|normal| and h = max_i |n_i| is the
+ * spacing of lattice planes with that normal.  xi is [>=ns][N1][N2][N3][2].
+ * Returns the number of points set. */
+long pfdd_add_loop(double *xi, int n1, int n2, int n3, int sys, const double normal[3],
+                   const double center[3], double radius);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""pfdd_amd -- Python view of libpfdd_gpu.so, the MI355X (gfx950) PFDD
+FFT-elasticity + Ginzburg-Landau step.
+
+The product is the C ABI in include/pfdd_gpu.h; this module binds it with
+ctypes for tests, the benchmark and scripting.  It never falls back to CPU
+code: if the library is missing, load() raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libpfdd_gpu.so")
+
+MAX_NS = 16
+OK, E_HIP, E_ALLOC, E_SHAPE, E_RADIX, E_NAN, E_UNSUPPORTED, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7
+FP64, MIXED = 0, 1
+B_STORED, B_RECOMPUTE = 0, 1
+FORWARD, BACKWARD = -1, 1
+
+EXPORTED = [
+    "pfdd_create", "pfdd_destroy", "pfdd_set_tau", "pfdd_upload_xi", "pfdd_download_xi",
+    "pfdd_begin_outer", "pfdd_gl_steps", "pfdd_strain_stress", "pfdd_fft", "pfdd_sync", "pfdd_stream",
+    "pfdd_device_bytes", "pfdd_strerror", "pfdd_setfcc", "pfdd_setorient", "pfdd_seteps",
+    "pfdd_resolve_tau", "pfdd_splitmix64", "pfdd_uniform01", "pfdd_add_loop", "pfdd_enqueue_stage",
+]
+STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY = range(7)
+
+
+class PfddError(RuntimeError):
+    def __init__(self, rc, what):
+        self.rc = rc
+        super().__init__(f"{what}: {strerror(rc)} ({rc})")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n1", ctypes.c_int), ("n2", ctypes.c_int), ("n3", ctypes.c_int), ("ns", ctypes.c_int),
+        ("precision", ctypes.c_int), ("bmode", ctypes.c_int),
+        ("C11", ctypes.c_double), ("C12", ctypes.c_double), ("C44", ctypes.c_double),
+        ("d1", ctypes.c_double), ("d2", ctypes.c_double), ("d3", ctypes.c_double),
+        ("cd2", ctypes.c_double * MAX_NS), ("asf2", ctypes.c_double * MAX_NS),
+        ("dslip", ctypes.c_double * MAX_NS),
+        ("uq2", ctypes.c_double), ("pi_const", ctypes.c_double),
+        ("eps", ctypes.c_double * (MAX_NS * 9)),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load libpfdd_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+    L.pfdd_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.pfdd_destroy.argtypes = [vp]
+    L.pfdd_destroy.restype = None
+    L.pfdd_set_tau.argtypes = [vp, vp, ci]
+    L.pfdd_upload_xi.argtypes = [vp, vp, vp, ci]
+    L.pfdd_download_xi.argtypes = [vp, vp, ci]
+    L.pfdd_begin_outer.argtypes = [vp, ci]
+    L.pfdd_gl_steps.argtypes = [vp, ci, vp, vp]
+    L.pfdd_strain_stress.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.pfdd_fft.argtypes = [vp, vp, ci, ci, ci]
+    L.pfdd_sync.argtypes = [vp]
+    L.pfdd_stream.argtypes = [vp]
+    L.pfdd_stream.restype = vp
+    L.pfdd_device_bytes.argtypes = [vp]
+    L.pfdd_device_bytes.restype = ctypes.c_longlong
+    L.pfdd_strerror.argtypes = [ci]
+    L.pfdd_strerror.restype = ctypes.c_char_p
+    L.pfdd_setfcc.argtypes = [vp, vp]
+    L.pfdd_setfcc.restype = None
+    L.pfdd_setorient.argtypes = [vp, vp]
+    L.pfdd_setorient.restype = None
+    L.pfdd_seteps.argtypes = [ci, vp, vp, vp, vp, vp]
+    L.pfdd_seteps.restype = None
+    L.pfdd_resolve_tau.argtypes = [ci, vp, vp, vp]
+    L.pfdd_resolve_tau.restype = None
+    L.pfdd_splitmix64.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
+    L.pfdd_splitmix64.restype = ctypes.c_ulonglong
+    L.pfdd_uniform01.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
+    L.pfdd_uniform01.restype = cd
+    L.pfdd_add_loop.argtypes = [vp, ci, ci, ci, ci, vp, vp, cd]
+    L.pfdd_add_loop.restype = cl
+    L.pfdd_enqueue_stage.argtypes = [vp, ci, ci]
+    _lib = L
+    return L
+
+
+def strerror(rc):
+    return load().pfdd_strerror(rc).decode()
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise PfddError(rc, what)
+
+
+def make_params(shape, eps, C11, C12, C44, cd2, asf2, dslip, precision=FP64, bmode=B_RECOMPUTE,
+                d=(1.0, 1.0, 1.0), uq2=1.0, pi_const=3.141592654):
+    p = Params()
+    p.n1, p.n2, p.n3 = (int(v) for v in shape)
+    eps = np.asarray(eps, dtype=np.float64).reshape(-1, 9)
+    p.ns = eps.shape[0]
+    p.precision, p.bmode = precision, bmode
+    p.C11, p.C12, p.C44 = C11, C12, C44
+    p.d1, p.d2, p.d3 = d
+    for a in range(p.ns):
+        p.cd2[a], p.asf2[a], p.dslip[a] = cd2[a], asf2[a], dslip[a]
+        for t in range(9):
+            p.eps[9 * a + t] = eps[a, t]
+    p.uq2, p.pi_const = uq2, pi_const
+    return p
+
+
+class Context:
+    """One pfdd_ctx: device state of one PFDD instance on the current GPU."""
+
+    def __init__(self, params):
+        L = load()
+        self.params = params
+        self.ns = params.ns
+        self.shape = (params.n1, params.n2, params.n3)
+        self.P = params.n1 * params.n2 * params.n3
+        h = ctypes.c_void_p()
+        _check(L.pfdd_create(ctypes.byref(params), ctypes.byref(h)), "pfdd_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pfdd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tau(self, tau, per_point=False):
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        _check(load().pfdd_set_tau(self.h, t.ctypes.data, 1 if per_point else 0), "pfdd_set_tau")
+
+    def upload_xi(self, xi, xi_bc=None):
+        """xi: complex [nsv, N1, N2, N3] (nsv >= ns; fields past ns must be zero)."""
+        x = np.ascontiguousarray(xi, dtype=np.complex128)
+        nsv = x.shape[0]
+        b = None if xi_bc is None else np.ascontiguousarray(xi_bc, dtype=np.complex128)
+        _check(load().pfdd_upload_xi(self.h, x.ctypes.data, None if b is None else b.ctypes.data, nsv),
+               "pfdd_upload_xi")
+
+    def download_xi(self, nsv=None):
+        nsv = self.ns if nsv is None else nsv
+        x = np.zeros((nsv,) + self.shape, dtype=np.complex128)
+        _check(load().pfdd_download_xi(self.h, x.ctypes.data, nsv), "pfdd_download_xi")
+        return x
+
+    def begin_outer(self, quirk=True):
+        _check(load().pfdd_begin_outer(self.h, 1 if quirk else 0), "pfdd_begin_outer")
+
+    def gl_steps(self, n, record=True):
+        if not record:
+            _check(load().pfdd_gl_steps(self.h, n, None, None), "pfdd_gl_steps")
+            return None, None
+        g = np.zeros(n)
+        ga = np.zeros((n, self.ns))
+        _check(load().pfdd_gl_steps(self.h, n, g.ctypes.data, ga.ctypes.data), "pfdd_gl_steps")
+        return g, ga
+
+    def strain_stress(self, avepst_uniform, fields=False):
+        ap = np.ascontiguousarray(avepst_uniform, dtype=np.float64).reshape(9)
+        aveps, avesig = np.zeros(9), np.zeros(9)
+        ef = sf = None
+        if fields:
+            ef = np.zeros((3, 3) + self.shape)
+            sf = np.zeros((3, 3) + self.shape)
+        _check(load().pfdd_strain_stress(self.h, ap.ctypes.data, aveps.ctypes.data, avesig.ctypes.data,
+                                         None if ef is None else ef.ctypes.data,
+                                         None if sf is None else sf.ctypes.data), "pfdd_strain_stress")
+        if fields:
+            return aveps.reshape(3, 3), avesig.reshape(3, 3), ef, sf
+        return aveps.reshape(3, 3), avesig.reshape(3, 3)
+
+    def fft(self, dev_ptr, batch, direction, precision):
+        _check(load().pfdd_fft(self.h, dev_ptr, batch, direction, precision), "pfdd_fft")
+
+    def sync(self):
+        _check(load().pfdd_sync(self.h), "pfdd_sync")
+
+    def enqueue_stage(self, stage, reps):
+        _check(load().pfdd_enqueue_stage(self.h, stage, reps), "pfdd_enqueue_stage")
+
+    @property
+    def stream(self):
+        return load().pfdd_stream(self.h)
+
+    @property
+    def device_bytes(self):
+        return load().pfdd_device_bytes(self.h)

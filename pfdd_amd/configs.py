@@ -1,0 +1,218 @@
+"""Seeded benchmark workloads (PFDD GL loop instances), built with the
+library's own setup helpers so every consumer sees bit-identical inputs.
+
+Shared setup: normalised isotropic elasticity (mu = C44 = 1, nu = 0.33,
+lambda = C12 = 2 nu/(1-2 nu), C11 = lambda + 2 mu), unit grid spacing,
+CD2 = 0.1, dslip = 1, uq2 = 1, pi = 3.141592654, Asf2 as main() sets it
+(G.c:4278-4285).  Random draws come from SplitMix64 with base seed 20261202
+(+ config index); per loop the draws are radius, system, centre x, y, z, and
+the applied stress takes six more draws for (11, 12, 13, 22, 23, 33).
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import FP64, MIXED, B_RECOMPUTE, B_STORED, load, make_params
+
+BASE_SEED = 20261202
+NU = 0.33
+MU = 1.0
+LAM = 2 * NU / (1 - 2 * NU)
+C11, C12, C44 = LAM + 2 * MU, LAM, MU
+PI_REF = 3.141592654
+
+
+def asf2_reference(ns):
+    a = [0.0262] * ns
+    for i in range(min(2, ns)):
+        a[i] = 0.0064659518668288441
+    if ns == 4:
+        a[2] = a[3] = 0.013307048604153731
+    return a
+
+
+def setfcc():
+    xn = np.zeros((12, 3))
+    xb = np.zeros((12, 3))
+    load().pfdd_setfcc(xn.ctypes.data, xb.ctypes.data)
+    return xn, xb
+
+
+def setorient(rows):
+    r = np.ascontiguousarray(rows, dtype=np.float64)
+    th = np.zeros((3, 3))
+    load().pfdd_setorient(r.ctypes.data, th.ctypes.data)
+    return th
+
+
+def seteps(xn, xb, dslip, theta):
+    ns = len(xn)
+    xn = np.ascontiguousarray(xn, dtype=np.float64)
+    xb = np.ascontiguousarray(xb, dtype=np.float64)
+    ds = np.ascontiguousarray(dslip, dtype=np.float64)
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    eps = np.zeros((ns, 3, 3))
+    load().pfdd_seteps(ns, xn.ctypes.data, xb.ctypes.data, ds.ctypes.data, th.ctypes.data, eps.ctypes.data)
+    return eps
+
+
+def resolve_tau(sigma, eps):
+    s = np.ascontiguousarray(sigma, dtype=np.float64)
+    e = np.ascontiguousarray(eps, dtype=np.float64)
+    tau = np.zeros(len(e))
+    load().pfdd_resolve_tau(len(e), s.ctypes.data, e.ctypes.data, tau.ctypes.data)
+    return tau
+
+
+class Rng:
+    def __init__(self, seed):
+        self.s = ctypes.c_ulonglong(seed)
+
+    def uniform(self):
+        return load().pfdd_uniform01(ctypes.byref(self.s))
+
+    def next64(self):
+        return load().pfdd_splitmix64(ctypes.byref(self.s))
+
+
+def add_loop(xi, sys, normal, center, radius):
+    ns_, n1, n2, n3 = xi.shape
+    assert xi.dtype == np.complex128 and xi.flags.c_contiguous
+    nrm = np.ascontiguousarray(normal, dtype=np.float64)
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    return load().pfdd_add_loop(xi.ctypes.data, n1, n2, n3, sys, nrm.ctypes.data, c.ctypes.data, float(radius))
+
+
+# Orientation whose rows are [-110]/sqrt2, [-1-12]/sqrt6, [111]/sqrt3: the
+# (111) normal maps onto z, so (111) loops lie in grid planes k = const.
+ROWS_111_TO_Z = [[-1 / np.sqrt(2), 1 / np.sqrt(2), 0.0],
+                 [-1 / np.sqrt(6), -1 / np.sqrt(6), 2 / np.sqrt(6)],
+                 [1 / np.sqrt(3), 1 / np.sqrt(3), 1 / np.sqrt(3)]]
+
+
+@dataclass
+class Workload:
+    name: str
+    shape: tuple
+    eps: np.ndarray          # [ns,3,3]
+    normals: np.ndarray      # [ns,3] slip-plane normals in grid coordinates
+    xi: np.ndarray           # complex [ns, N1, N2, N3]
+    tau: np.ndarray          # [ns]
+    sigma: np.ndarray        # [3,3] applied stress / mu
+    steps: int
+    precision: int = FP64
+    bmode: int = B_RECOMPUTE
+    loops: list = field(default_factory=list)
+
+    @property
+    def ns(self):
+        return self.eps.shape[0]
+
+    @property
+    def P(self):
+        return int(np.prod(self.shape))
+
+    def params(self, precision=None, bmode=None):
+        ns = self.ns
+        return make_params(self.shape, self.eps, C11, C12, C44, [0.1] * ns, asf2_reference(ns), [1.0] * ns,
+                           precision=self.precision if precision is None else precision,
+                           bmode=self.bmode if bmode is None else bmode, pi_const=PI_REF)
+
+    def s_tensor_avepst(self):
+        """S:sigma*mu, the xi-independent part of avepst (avestrain G.c:714-729)."""
+        S11 = 1. / 3. * (1 / (C11 + 2 * C12) + 2 / (C11 - C12))
+        S12 = 1. / 3. * (1 / (C11 + 2 * C12) - 1 / (C11 - C12))
+        S44 = 1 / C44
+        d = np.eye(3)
+        out = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                v = 0.0
+                for k in range(3):
+                    for l in range(3):
+                        d4 = 1.0 if i == j == k == l else 0.0
+                        Sijkl = ((S11 + 2 * S12) / 3.0 * d[i, j] * d[k, l]
+                                 + S44 / 4.0 * (d[i, k] * d[j, l] + d[i, l] * d[j, k])
+                                 - S44 / 2.0 * d4 + (S11 - S12) * (d4 - d[i, j] * d[k, l] / 3.0))
+                        v += Sijkl * self.sigma[k, l] * MU
+                out[i, j] = v
+        return out
+
+
+def _fcc_systems(systems, theta):
+    xn, xb = setfcc()
+    xn, xb = xn[systems], xb[systems]
+    eps = seteps(xn, xb, [1.0] * len(systems), theta)
+    normals = (theta @ xn.T).T
+    return eps, normals
+
+
+def _random_loops(rng, xi, normals, nloops, rmin, rmax):
+    ns, n1, n2, n3 = xi.shape
+    loops = []
+    for _ in range(nloops):
+        r = rmin + (rmax - rmin) * rng.uniform()
+        a = min(int(rng.uniform() * ns), ns - 1)
+        c = (rng.uniform() * n1, rng.uniform() * n2, rng.uniform() * n3)
+        cnt = add_loop(xi, a, normals[a], c, r)
+        loops.append((a, c, r, cnt))
+    return loops
+
+
+def _random_sigma(rng, amp):
+    s = np.zeros((3, 3))
+    for (i, j) in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)):
+        s[i, j] = s[j, i] = -amp + 2 * amp * rng.uniform()
+    return s
+
+
+def c0():
+    """32^3, NS=3: the three a/2<110> systems of one (111) plane, crystal
+    rotated so the plane is z = const; one loop of radius 8 on plane z = 16
+    (centre (16,16,16)) on system 0; sigma gives tau_0 = 0.01 mu."""
+    shape = (32, 32, 32)
+    theta = setorient(ROWS_111_TO_Z)
+    eps, normals = _fcc_systems([0, 1, 2], theta)
+    xi = np.zeros((3,) + shape, dtype=np.complex128)
+    cnt = add_loop(xi, 0, normals[0], (16.0, 16.0, 16.0), 8.0)
+    e0 = eps[0]
+    se = 0.5 * (e0 + e0.T)
+    sigma = 0.01 * se / np.sum(se * e0)
+    return Workload("c0", shape, eps, normals, xi, resolve_tau(sigma, eps), sigma, 500,
+                    loops=[(0, (16.0, 16.0, 16.0), 8.0, cnt)])
+
+
+def _fcc_random(name, shape, nloops, seed, steps, precision, bmode):
+    theta = setorient(np.eye(3))
+    eps, normals = _fcc_systems(list(range(12)), theta)
+    xi = np.zeros((12,) + tuple(shape), dtype=np.complex128)
+    rng = Rng(seed)
+    loops = _random_loops(rng, xi, normals, nloops, 5.0, 20.0)
+    sigma = _random_sigma(rng, 0.02)
+    return Workload(name, tuple(shape), eps, normals, xi, resolve_tau(sigma, eps), sigma, steps, precision, bmode,
+                    loops)
+
+
+def c1(bmode=B_RECOMPUTE):
+    """128^3, NS=12 (FCC), 16 seeded loops r ~ U[5,20], sigma_ij ~ U[-0.02,0.02]."""
+    return _fcc_random("c1", (128, 128, 128), 16, BASE_SEED + 1, 1000, FP64, bmode)
+
+
+def c2(bmode=B_STORED):
+    """256^3, NS=12 (FCC), 64 seeded loops, mixed precision."""
+    return _fcc_random("c2", (256, 256, 256), 64, BASE_SEED + 2, 2000, MIXED, bmode)
+
+
+def small_fcc(shape, nloops=4, seed=BASE_SEED + 100, ns=12, rmin=2.0, rmax=6.0):
+    """A reduced FCC instance with the c1 generator, for parity tests."""
+    theta = setorient(np.eye(3))
+    eps, normals = _fcc_systems(list(range(ns)), theta)
+    xi = np.zeros((ns,) + tuple(shape), dtype=np.complex128)
+    rng = Rng(seed)
+    loops = _random_loops(rng, xi, normals, nloops, rmin, rmax)
+    sigma = _random_sigma(rng, 0.02)
+    return Workload(f"fcc{shape}", tuple(shape), eps, normals, xi, resolve_tau(sigma, eps), sigma, 10, loops=loops)
+
+
+BY_NAME = {"c0": c0, "c1": c1, "c2": c2}

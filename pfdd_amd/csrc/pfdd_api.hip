@@ -1,0 +1,667 @@
+// pfdd_api.hip -- the C ABI of libpfdd_gpu.so (include/pfdd_gpu.h).
+//
+// One pfdd_ctx owns its device memory and one HIP stream (default priority).
+// A GL step is: k_update (plasticevolv + copy to the FFT array + gamma sums),
+// forward 3D FFT, contraction with B(k), inverse 3D FFT.  Chunks of steps
+// are captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "pfdd_gpu.h"
+#include "pfdd_kernels.hpp"
+
+using namespace pfdd;
+
+#define HIPCK(x)                                   \
+  do {                                             \
+    hipError_t e_ = (x);                           \
+    if (e_ != hipSuccess) return PFDD_E_HIP;       \
+  } while (0)
+
+namespace {
+
+constexpr int GRAPH_STEPS = 8;      // GL steps per captured graph
+constexpr int HIST_CAP = 1024;      // gamma history rows per chunk
+
+struct Twiddles {
+  int n = 0;
+  double2* d = nullptr;
+  float2* f = nullptr;
+};
+
+}  // namespace
+
+struct pfdd_ctx {
+  pfdd_params p{};
+  long P = 0;
+  int S = 0;
+  int ntri = 0;
+  int fp64 = 1;
+  int stored = 0;
+  hipStream_t stream = nullptr;
+  // device buffers
+  double2* xi = nullptr;
+  void* W = nullptr;         // cx<T> [S][P]
+  void* Bsym = nullptr;      // T [ntri][P]
+  uint8_t* mask = nullptr;
+  double* taupp = nullptr;
+  double* partials = nullptr;
+  size_t partials_len = 0;
+  double* hist = nullptr;    // [HIST_CAP][S+1]
+  unsigned* counter = nullptr;
+  int* step_ctr = nullptr;
+  double* small = nullptr;   // reduction outputs (<= 32 doubles)
+  double* avepst_d = nullptr;
+  ElasticConsts* ec = nullptr;
+  void* E6 = nullptr;        // strain spectrum [6][P], allocated on first use
+  double* efield = nullptr;  // [9][P] strain/stress fields, on demand
+  double* sfield = nullptr;
+  Twiddles tw[3];
+  StepArgs sa{};
+  int upd_nblk = 0;
+  int has_xi = 0;
+  int prologue_done = 0;
+  hipGraphExec_t graph = nullptr;
+  long long bytes = 0;
+};
+
+namespace {
+
+constexpr int kLens[] = {2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+
+bool len_ok(int n) {
+  for (int v : kLens)
+    if (v == n) return true;
+  return false;
+}
+
+template <typename T> const cx<T>* tw_of(const Twiddles& t);
+template <> const double2* tw_of<double>(const Twiddles& t) { return t.d; }
+template <> const float2* tw_of<float>(const Twiddles& t) { return t.f; }
+
+template <typename T, int N>
+void launch_axis_n(cx<T>* data, long outer, long inner, int sign, const cx<T>* tw, hipStream_t st) {
+  constexpr int TPP = make_plan(N).tpp;
+  if (inner == 1) {
+    constexpr int TILE = tile_rows<T, N>();
+    const long nrows = outer;
+    const long nb = (nrows + TILE - 1) / TILE;
+    hipLaunchKernelGGL((k_fft_rows<T, N>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, nrows, (T)sign, tw);
+  } else {
+    constexpr int TILE = tile_cols<T, N>();
+    int tile = 1;
+    while (tile * 2 <= TILE && inner % (tile * 2) == 0) tile *= 2;
+    const long nb = outer * (inner / tile);
+    hipLaunchKernelGGL((k_fft_cols<T, N>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, inner, tile, (T)sign, tw);
+  }
+}
+
+// Transform along one axis of an [outer][n][inner] complex array.
+template <typename T>
+int launch_axis(cx<T>* data, long outer, int n, long inner, int sign, const cx<T>* tw, hipStream_t st) {
+  switch (n) {
+#define PFDD_LEN(NN) case NN: launch_axis_n<T, NN>(data, outer, inner, sign, tw, st); break;
+    PFDD_LEN(2) PFDD_LEN(3) PFDD_LEN(4) PFDD_LEN(6) PFDD_LEN(8) PFDD_LEN(12) PFDD_LEN(16) PFDD_LEN(24)
+    PFDD_LEN(32) PFDD_LEN(48) PFDD_LEN(64) PFDD_LEN(96) PFDD_LEN(128) PFDD_LEN(192) PFDD_LEN(256)
+    PFDD_LEN(384) PFDD_LEN(512)
+#undef PFDD_LEN
+    default: return PFDD_E_RADIX;
+  }
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+template <typename T>
+int fft3d(pfdd_ctx* c, cx<T>* data, int batch, int sign) {
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3;
+  int rc;
+  if ((rc = launch_axis<T>(data, (long)batch * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream))) return rc;
+  if ((rc = launch_axis<T>(data, (long)batch * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream))) return rc;
+  if ((rc = launch_axis<T>(data, (long)batch, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream))) return rc;
+  return PFDD_OK;
+}
+
+template <typename T>
+int contract(pfdd_ctx* c, int quirk) {
+  const unsigned nb = (unsigned)((c->P + 255) / 256);
+  cx<T>* W = (cx<T>*)c->W;
+  const T* B = (const T*)c->Bsym;
+#define PFDD_CT(SM)                                                                                     \
+  if (c->stored)                                                                                        \
+    hipLaunchKernelGGL((k_contract<T, SM, true>), dim3(nb), dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
+  else                                                                                                  \
+    hipLaunchKernelGGL((k_contract<T, SM, false>), dim3(nb), dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
+  if (c->S <= 4) { PFDD_CT(4) } else if (c->S <= 8) { PFDD_CT(8) } else { PFDD_CT(16) }
+#undef PFDD_CT
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+template <typename T>
+int update(pfdd_ctx* c) {
+  dim3 grid((unsigned)c->upd_nblk, (unsigned)c->S);
+  cx<T>* W = (cx<T>*)c->W;
+  const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+#define PFDD_UP(MM, TT)                                                                                  \
+  hipLaunchKernelGGL((k_update<T, MM, TT>), grid, dim3(UPD_THREADS), 0, c->stream, c->xi, W, c->mask, c->taupp, \
+                     c->sa, c->partials, c->counter, c->hist, c->step_ctr)
+  if (m && tp) PFDD_UP(true, true);
+  else if (m) PFDD_UP(true, false);
+  else if (tp) PFDD_UP(false, true);
+  else PFDD_UP(false, false);
+#undef PFDD_UP
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+template <typename T>
+int elastic_solve(pfdd_ctx* c, int quirk) {
+  int rc;
+  cx<T>* W = (cx<T>*)c->W;
+  if ((rc = fft3d<T>(c, W, c->S, PFDD_FORWARD))) return rc;
+  if ((rc = contract<T>(c, quirk))) return rc;
+  return fft3d<T>(c, W, c->S, PFDD_BACKWARD);
+}
+
+template <typename T>
+int one_step(pfdd_ctx* c) {
+  int rc;
+  if ((rc = update<T>(c))) return rc;
+  return elastic_solve<T>(c, 0);
+}
+
+int step(pfdd_ctx* c) { return c->fp64 ? one_step<double>(c) : one_step<float>(c); }
+
+int make_twiddles(Twiddles& t, int n, long long& bytes, hipStream_t st) {
+  static const long double TWO_PI = 6.283185307179586476925286766559005768L;
+  std::vector<double2> hd(n);
+  std::vector<float2> hf(n);
+  for (int k = 0; k < n; k++) {
+    const long double a = TWO_PI * (long double)k / (long double)n;
+    hd[k] = make_double2((double)cosl(a), (double)sinl(a));
+    hf[k] = make_float2((float)hd[k].x, (float)hd[k].y);
+  }
+  t.n = n;
+  HIPCK(hipMalloc(&t.d, sizeof(double2) * n));
+  HIPCK(hipMalloc(&t.f, sizeof(float2) * n));
+  HIPCK(hipMemcpyAsync(t.d, hd.data(), sizeof(double2) * n, hipMemcpyHostToDevice, st));
+  HIPCK(hipMemcpyAsync(t.f, hf.data(), sizeof(float2) * n, hipMemcpyHostToDevice, st));
+  HIPCK(hipStreamSynchronize(st));
+  bytes += (long long)n * 24;
+  return PFDD_OK;
+}
+
+void cijkl(double C[81], double C11, double C12, double C44) {
+  const double mu = C44, ll = C12, mup = C11 - C12 - 2 * C44;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int m = 0; m < 3; m++) {
+          const double dik = i == k, djm = j == m, dim = i == m, djk = j == k, dij = i == j, dkm = k == m;
+          const double d4 = (i == j && j == k && k == m);
+          C[((i * 3 + j) * 3 + k) * 3 + m] = mu * (dik * djm + dim * djk) + ll * dij * dkm + mup * d4;
+        }
+}
+
+template <typename T>
+void* dev_alloc(size_t bytes, long long& total) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  total += (long long)bytes;
+  return p;
+}
+
+int ensure_strain_buffers(pfdd_ctx* c, bool fields) {
+  const size_t esz = c->fp64 ? sizeof(double2) : sizeof(float2);
+  if (!c->E6) {
+    c->E6 = dev_alloc<char>(esz * 6 * (size_t)c->P, c->bytes);
+    if (!c->E6) return PFDD_E_ALLOC;
+  }
+  if (fields && !c->efield) {
+    c->efield = (double*)dev_alloc<char>(sizeof(double) * 9 * (size_t)c->P, c->bytes);
+    c->sfield = (double*)dev_alloc<char>(sizeof(double) * 9 * (size_t)c->P, c->bytes);
+    if (!c->efield || !c->sfield) return PFDD_E_ALLOC;
+  }
+  return PFDD_OK;
+}
+
+template <typename T>
+int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps, double* avesigma,
+                       double* eps_field, double* sig_field) {
+  int rc;
+  const bool fields = eps_field || sig_field;
+  if ((rc = ensure_strain_buffers(c, fields))) return rc;
+  cx<T>* W = (cx<T>*)c->W;
+  cx<T>* E = (cx<T>*)c->E6;
+  const long P = c->P;
+  const int S = c->S;
+  // 1. <xi_a> (avestrain G.c:670-686)
+  {
+    dim3 grid((unsigned)c->upd_nblk, (unsigned)S);
+    hipLaunchKernelGGL(k_xi_sums, grid, dim3(UPD_THREADS), 0, c->stream, c->xi, P, S, c->upd_nblk, c->partials,
+                       c->counter, c->small);
+    HIPCK(hipPeekAtLastError());
+  }
+  double sums[MAXS + 1];
+  HIPCK(hipMemcpyAsync(sums, c->small, sizeof(double) * (S + 1), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  double avepst[9];
+  for (int ij = 0; ij < 9; ij++) {
+    double v = avepst_uniform[ij];
+    for (int a = 0; a < S; a++) v += c->p.eps[a][ij / 3][ij % 3] * (sums[a] / (double)P);
+    avepst[ij] = v;
+  }
+  HIPCK(hipMemcpyAsync(c->avepst_d, avepst, sizeof(avepst), hipMemcpyHostToDevice, c->stream));
+  // 2. spectrum of xi, strain spectrum, 6 inverse transforms (the work array
+  //    is rebuilt from xi afterwards, so the GL state is left untouched)
+  const unsigned nb = (unsigned)((P + 255) / 256);
+  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((S * P + 255) / 256)), dim3(256), 0, c->stream, c->xi, W, S * P);
+  if ((rc = fft3d<T>(c, W, S, PFDD_FORWARD))) return rc;
+#define PFDD_SS(SM) hipLaunchKernelGGL((k_strain_spectrum<T, SM>), dim3(nb), dim3(256), 0, c->stream, W, E, c->ec, P)
+  if (S <= 4) PFDD_SS(4); else if (S <= 8) PFDD_SS(8); else PFDD_SS(16);
+#undef PFDD_SS
+  HIPCK(hipPeekAtLastError());
+  if ((rc = fft3d<T>(c, E, 6, PFDD_BACKWARD))) return rc;
+  // 3. per-point strain and stress, 18 device sums
+  double* ef = fields ? c->efield : nullptr;
+  double* sf = fields ? c->sfield : nullptr;
+#define PFDD_SR(SM)                                                                                       \
+  hipLaunchKernelGGL((k_strain_stress<T, SM>), dim3(nb), dim3(256), 0, c->stream, E, c->xi, c->ec, c->avepst_d, P, \
+                     c->partials, (int)nb, c->counter, c->small, ef, sf)
+  if (S <= 4) PFDD_SR(4); else if (S <= 8) PFDD_SR(8); else PFDD_SR(16);
+#undef PFDD_SR
+  HIPCK(hipPeekAtLastError());
+  double out[19];
+  HIPCK(hipMemcpyAsync(out, c->small, sizeof(double) * 19, hipMemcpyDeviceToHost, c->stream));
+  if (eps_field) HIPCK(hipMemcpyAsync(eps_field, ef, sizeof(double) * 9 * P, hipMemcpyDeviceToHost, c->stream));
+  if (sig_field) HIPCK(hipMemcpyAsync(sig_field, sf, sizeof(double) * 9 * P, hipMemcpyDeviceToHost, c->stream));
+  // restore the GL state: W must again hold the inverse-transformed data2
+  // (the prologue has to be rerun by the caller if it had been done)
+  HIPCK(hipStreamSynchronize(c->stream));
+  for (int ij = 0; ij < 9; ij++) {
+    aveps[ij] = out[ij] / (double)P;
+    avesigma[ij] = out[9 + ij] / (double)P;
+  }
+  return PFDD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pfdd_strerror(int rc) {
+  switch (rc) {
+    case PFDD_OK: return "ok";
+    case PFDD_E_HIP: return "HIP runtime error";
+    case PFDD_E_ALLOC: return "allocation failed";
+    case PFDD_E_SHAPE: return "bad shape or argument";
+    case PFDD_E_RADIX: return "unsupported FFT length";
+    case PFDD_E_NAN: return "non-finite value";
+    case PFDD_E_UNSUPPORTED: return "unsupported input";
+    case PFDD_E_STATE: return "call out of order";
+    default: return "unknown error";
+  }
+}
+
+void pfdd_destroy(pfdd_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  void* bufs[] = {c->xi, c->W, c->Bsym, c->mask, c->taupp, c->partials, c->hist, c->counter, c->step_ctr,
+                  c->small, c->avepst_d, c->ec, c->E6, c->efield, c->sfield};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto& t : c->tw) {
+    if (t.d) (void)hipFree(t.d);
+    if (t.f) (void)hipFree(t.f);
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
+  if (!p || !out) return PFDD_E_SHAPE;
+  *out = nullptr;
+  if (p->ns < 1 || p->ns > PFDD_MAX_NS) return PFDD_E_SHAPE;
+  if (p->precision != PFDD_FP64 && p->precision != PFDD_MIXED) return PFDD_E_SHAPE;
+  if (p->bmode != PFDD_B_STORED && p->bmode != PFDD_B_RECOMPUTE) return PFDD_E_SHAPE;
+  const int ns3[3] = {p->n1, p->n2, p->n3};
+  for (int i = 0; i < 3; i++)
+    if (!len_ok(ns3[i])) return PFDD_E_RADIX;
+  const double* chk[] = {&p->C11, &p->C12, &p->C44, &p->d1, &p->d2, &p->d3, &p->uq2, &p->pi_const};
+  for (const double* v : chk)
+    if (!std::isfinite(*v)) return PFDD_E_NAN;
+  if (!(p->C44 > 0) || !(p->d1 > 0) || !(p->d2 > 0) || !(p->d3 > 0)) return PFDD_E_SHAPE;
+  for (int a = 0; a < p->ns; a++) {
+    if (!std::isfinite(p->cd2[a]) || !std::isfinite(p->asf2[a]) || !std::isfinite(p->dslip[a]) || p->dslip[a] == 0)
+      return PFDD_E_NAN;
+    for (int i = 0; i < 9; i++)
+      if (!std::isfinite(p->eps[a][i / 3][i % 3])) return PFDD_E_NAN;
+  }
+  pfdd_ctx* c = new (std::nothrow) pfdd_ctx();
+  if (!c) return PFDD_E_ALLOC;
+  c->p = *p;
+  c->S = p->ns;
+  c->P = (long)p->n1 * p->n2 * p->n3;
+  c->ntri = c->S * (c->S + 1) / 2;
+  c->fp64 = p->precision == PFDD_FP64;
+  c->stored = p->bmode == PFDD_B_STORED;
+  int rc = PFDD_OK;
+#define FAIL(code) do { rc = (code); goto fail; } while (0)
+  {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) FAIL(PFDD_E_HIP);
+    const size_t csz = c->fp64 ? sizeof(double2) : sizeof(float2);
+    const size_t rsz = c->fp64 ? sizeof(double) : sizeof(float);
+    const long P = c->P;
+    const int S = c->S;
+    c->upd_nblk = (int)((P + UPD_THREADS * UPD_EPT - 1) / (UPD_THREADS * UPD_EPT));
+    const long strain_nblk = (P + 255) / 256;
+    c->partials_len = (size_t)std::max<long>((long)S * c->upd_nblk, 18 * strain_nblk);
+    if (!(c->xi = (double2*)dev_alloc<char>(sizeof(double2) * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->W = dev_alloc<char>(csz * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->hist = (double*)dev_alloc<char>(sizeof(double) * HIST_CAP * (S + 1), c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->counter = (unsigned*)dev_alloc<char>(sizeof(unsigned) * 4, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->step_ctr = (int*)dev_alloc<char>(sizeof(int) * 4, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->small = (double*)dev_alloc<char>(sizeof(double) * 32, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->avepst_d = (double*)dev_alloc<char>(sizeof(double) * 9, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->ec = (ElasticConsts*)dev_alloc<char>(sizeof(ElasticConsts), c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 4, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipMemsetAsync(c->step_ctr, 0, sizeof(int) * 4, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipMemsetAsync(c->xi, 0, sizeof(double2) * S * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipMemsetAsync(c->W, 0, csz * S * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    // twiddle tables, one per axis
+    for (int i = 0; i < 3; i++)
+      if ((rc = make_twiddles(c->tw[i], ns3[i], c->bytes, c->stream))) goto fail;
+    // elastic constants (Bmatrix G.c:340-358; s_a = C:e_a etc.)
+    ElasticConsts h{};
+    h.mu = p->C44;
+    h.ll = p->C12;
+    h.mup = p->C11 - p->C12 - 2 * p->C44;
+    h.inv_mu = 1.0 / p->C44;
+    cijkl(h.C, p->C11, p->C12, p->C44);
+    for (int pq = 0; pq < 9; pq++) {
+      double t = 0.0;
+      for (int ij = 0; ij < 9; ij++) t += h.C[pq * 9 + ij];
+      h.T[pq] = t;
+    }
+    for (int a = 0; a < S; a++) {
+      for (int ij = 0; ij < 9; ij++) h.eps[a][ij] = p->eps[a][ij / 3][ij % 3];
+      double ts = 0.0;
+      for (int ij = 0; ij < 9; ij++) {
+        double s = 0.0;
+        for (int mn = 0; mn < 9; mn++) s += h.C[ij * 9 + mn] * h.eps[a][mn];
+        h.sig[a][ij] = s;
+        ts += s;
+      }
+      h.Tsum[a] = ts;
+    }
+    for (int a = 0; a < S; a++)
+      for (int b = 0; b < S; b++) {
+        double se = 0.0;
+        for (int mn = 0; mn < 9; mn++) se += h.sig[a][mn] * h.eps[b][mn];
+        h.SE[a][b] = se;
+      }
+    h.d[0] = p->d1; h.d[1] = p->d2; h.d[2] = p->d3;
+    h.n[0] = p->n1; h.n[1] = p->n2; h.n[2] = p->n3;
+    h.ns = S;
+    if (hipMemcpyAsync(c->ec, &h, sizeof(h), hipMemcpyHostToDevice, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    // step arguments
+    c->sa.P = P;
+    c->sa.ns = S;
+    c->sa.nblk = c->upd_nblk;
+    c->sa.uq2 = p->uq2;
+    c->sa.pi = p->pi_const;
+    for (int a = 0; a < S; a++) {
+      c->sa.cd2[a] = p->cd2[a];
+      c->sa.asf2[a] = p->asf2[a];
+      c->sa.dslip[a] = p->dslip[a];
+      c->sa.tau[a] = 0.0;
+    }
+    if (c->stored) {
+      if (!(c->Bsym = dev_alloc<char>(rsz * c->ntri * P, c->bytes))) FAIL(PFDD_E_ALLOC);
+      const unsigned nb = (unsigned)((P + 255) / 256);
+#define PFDD_BB(T, SM) hipLaunchKernelGGL((k_build_B<T, SM>), dim3(nb), dim3(256), 0, c->stream, (T*)c->Bsym, c->ec, P)
+      if (c->fp64) {
+        if (S <= 4) PFDD_BB(double, 4); else if (S <= 8) PFDD_BB(double, 8); else PFDD_BB(double, 16);
+      } else {
+        if (S <= 4) PFDD_BB(float, 4); else if (S <= 8) PFDD_BB(float, 8); else PFDD_BB(float, 16);
+      }
+#undef PFDD_BB
+      if (hipPeekAtLastError() != hipSuccess) FAIL(PFDD_E_HIP);
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+  }
+#undef FAIL
+  *out = c;
+  return PFDD_OK;
+fail:
+  pfdd_destroy(c);
+  return rc;
+}
+
+int pfdd_set_tau(pfdd_ctx* c, const double* tau, int per_point) {
+  if (!c || !tau) return PFDD_E_SHAPE;
+  const int S = c->S;
+  if (!per_point) {
+    for (int a = 0; a < S; a++) {
+      if (!std::isfinite(tau[a])) return PFDD_E_NAN;
+      c->sa.tau[a] = tau[a];
+    }
+    if (c->taupp) {
+      HIPCK(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->taupp);
+      c->taupp = nullptr;
+    }
+  } else {
+    const size_t n = (size_t)c->P * S;
+    for (size_t i = 0; i < n; i++)
+      if (!std::isfinite(tau[i])) return PFDD_E_NAN;
+    if (!c->taupp && !(c->taupp = (double*)dev_alloc<char>(sizeof(double) * n, c->bytes))) return PFDD_E_ALLOC;
+    HIPCK(hipMemcpyAsync(c->taupp, tau, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
+  // the captured graph holds the old kernel arguments
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  return PFDD_OK;
+}
+
+int pfdd_upload_xi(pfdd_ctx* c, const double* xi, const double* xi_bc, int nsv) {
+  if (!c || !xi || nsv < c->S) return PFDD_E_SHAPE;
+  const long P = c->P;
+  const int S = c->S;
+  for (long i = 0; i < 2 * (long)S * P; i++)
+    if (!std::isfinite(xi[i])) return PFDD_E_NAN;
+  for (long i = 2 * (long)S * P; i < 2 * (long)nsv * P; i++)
+    if (xi[i] != 0.0) return PFDD_E_UNSUPPORTED;
+  bool any_frozen = false;
+  std::vector<uint8_t> m;
+  if (xi_bc) {
+    for (long i = 0; i < (long)S * P && !any_frozen; i++) any_frozen = xi_bc[2 * i] != 0.0;
+    if (any_frozen) {
+      m.resize((size_t)S * P);
+      for (long i = 0; i < (long)S * P; i++) m[i] = xi_bc[2 * i] != 0.0;
+    }
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
+  HIPCK(hipMemcpyAsync(c->xi, xi, sizeof(double2) * S * P, hipMemcpyHostToDevice, c->stream));
+  if (any_frozen) {
+    if (!c->mask && !(c->mask = (uint8_t*)dev_alloc<char>((size_t)S * P, c->bytes))) return PFDD_E_ALLOC;
+    HIPCK(hipMemcpyAsync(c->mask, m.data(), (size_t)S * P, hipMemcpyHostToDevice, c->stream));
+  } else if (c->mask) {
+    (void)hipFree(c->mask);
+    c->mask = nullptr;
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  c->has_xi = 1;
+  c->prologue_done = 0;
+  return PFDD_OK;
+}
+
+int pfdd_download_xi(pfdd_ctx* c, double* xi, int nsv) {
+  if (!c || !xi || nsv < c->S) return PFDD_E_SHAPE;
+  const long P = c->P;
+  HIPCK(hipStreamSynchronize(c->stream));
+  HIPCK(hipMemcpyAsync(xi, c->xi, sizeof(double2) * c->S * P, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (nsv > c->S) memset(xi + 2 * (long)c->S * P, 0, sizeof(double) * 2 * (size_t)(nsv - c->S) * P);
+  return PFDD_OK;
+}
+
+int pfdd_begin_outer(pfdd_ctx* c, int quirk) {
+  if (!c) return PFDD_E_SHAPE;
+  if (!c->has_xi) return PFDD_E_STATE;
+  const long n = (long)c->S * c->P;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  int rc;
+  if (c->fp64) {
+    hipLaunchKernelGGL((k_xi_to_w<double>), dim3(nb), dim3(256), 0, c->stream, c->xi, (double2*)c->W, n);
+    rc = elastic_solve<double>(c, quirk ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL((k_xi_to_w<float>), dim3(nb), dim3(256), 0, c->stream, c->xi, (float2*)c->W, n);
+    rc = elastic_solve<float>(c, quirk ? 1 : 0);
+  }
+  if (rc) return rc;
+  c->prologue_done = 1;
+  return PFDD_OK;
+}
+
+int pfdd_gl_steps(pfdd_ctx* c, int nsteps, double* gamma, double* gamma_a) {
+  if (!c || nsteps < 0) return PFDD_E_SHAPE;
+  if (!c->prologue_done) return PFDD_E_STATE;
+  const int S = c->S;
+  std::vector<double> h;
+  int done = 0;
+  while (done < nsteps) {
+    const int chunk = std::min(HIST_CAP, nsteps - done);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
+    HIPCK(hipPeekAtLastError());
+    int left = chunk;
+    if (left >= GRAPH_STEPS && !c->graph) {
+      hipGraph_t g;
+      HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = PFDD_OK;
+      for (int s = 0; s < GRAPH_STEPS && rc == PFDD_OK; s++) rc = step(c);
+      hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (rc) return rc;
+      HIPCK(ec);
+      ec = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCK(ec);
+    }
+    while (left >= GRAPH_STEPS && c->graph) {
+      HIPCK(hipGraphLaunch(c->graph, c->stream));
+      left -= GRAPH_STEPS;
+    }
+    while (left > 0) {
+      int rc = step(c);
+      if (rc) return rc;
+      left--;
+    }
+    if (gamma || gamma_a) {
+      h.resize((size_t)chunk * (S + 1));
+      HIPCK(hipMemcpyAsync(h.data(), c->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(hipStreamSynchronize(c->stream));
+      for (int s = 0; s < chunk; s++) {
+        if (gamma) gamma[done + s] = h[(size_t)s * (S + 1) + S];
+        if (gamma_a)
+          for (int a = 0; a < S; a++) gamma_a[(size_t)(done + s) * S + a] = h[(size_t)s * (S + 1) + a];
+      }
+    }
+    done += chunk;
+  }
+  return PFDD_OK;
+}
+
+int pfdd_strain_stress(pfdd_ctx* c, const double* avepst_uniform, double* aveps, double* avesigma,
+                       double* eps_field, double* sig_field) {
+  if (!c || !avepst_uniform || !aveps || !avesigma) return PFDD_E_SHAPE;
+  if (!c->has_xi) return PFDD_E_STATE;
+  int rc = c->fp64 ? strain_stress_impl<double>(c, avepst_uniform, aveps, avesigma, eps_field, sig_field)
+                   : strain_stress_impl<float>(c, avepst_uniform, aveps, avesigma, eps_field, sig_field);
+  // the work array was reused; the next GL step needs a new prologue
+  c->prologue_done = 0;
+  return rc;
+}
+
+int pfdd_fft(pfdd_ctx* c, void* dev_buf, int batch, int dir, int prec) {
+  if (!c || !dev_buf || batch < 1) return PFDD_E_SHAPE;
+  if (dir != PFDD_FORWARD && dir != PFDD_BACKWARD) return PFDD_E_SHAPE;
+  if (prec == PFDD_FP64) return fft3d<double>(c, (double2*)dev_buf, batch, dir);
+  if (prec == PFDD_MIXED) return fft3d<float>(c, (float2*)dev_buf, batch, dir);
+  return PFDD_E_SHAPE;
+}
+
+int pfdd_sync(pfdd_ctx* c) {
+  if (!c) return PFDD_E_SHAPE;
+  HIPCK(hipStreamSynchronize(c->stream));
+  return PFDD_OK;
+}
+
+void* pfdd_stream(pfdd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+long long pfdd_device_bytes(const pfdd_ctx* c) { return c ? c->bytes : 0; }
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
+  cx<T>* W = (cx<T>*)c->W;
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
+  const long n = (long)S * c->P;
+  int rc = PFDD_OK;
+  for (int r = 0; r < reps && rc == PFDD_OK; r++) {
+    const int sign = (r & 1) ? PFDD_BACKWARD : PFDD_FORWARD;
+    switch (stage) {
+      case PFDD_STAGE_UPDATE:
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
+        rc = update<T>(c);
+        break;
+      case PFDD_STAGE_FFT_AXIS3:
+        rc = launch_axis<T>(W, (long)S * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream);
+        break;
+      case PFDD_STAGE_FFT_AXIS2:
+        rc = launch_axis<T>(W, (long)S * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream);
+        break;
+      case PFDD_STAGE_FFT_AXIS1:
+        rc = launch_axis<T>(W, (long)S, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream);
+        break;
+      case PFDD_STAGE_CONTRACT:
+        rc = contract<T>(c, 0);
+        break;
+      case PFDD_STAGE_STEP:
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
+        rc = one_step<T>(c);
+        break;
+      case PFDD_STAGE_COPY:
+        hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, W, n);
+        rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+        break;
+      default:
+        return PFDD_E_SHAPE;
+    }
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int pfdd_enqueue_stage(pfdd_ctx* c, int stage, int reps) {
+  if (!c || reps < 0 || stage < PFDD_STAGE_UPDATE || stage > PFDD_STAGE_COPY) return PFDD_E_SHAPE;
+  c->prologue_done = 0;
+  return c->fp64 ? enqueue_stage<double>(c, stage, reps) : enqueue_stage<float>(c, stage, reps);
+}

@@ -1,0 +1,181 @@
+// pfdd_fft.hpp -- device building blocks of the batched 3D C2C FFT.
+//
+// A 1D transform of length N runs on pencils staged in LDS: each pass of the
+// Stockham autosort algorithm (Govindaraju et al., SC08, "High performance
+// discrete Fourier transforms on graphics processors") reads R points per
+// butterfly from LDS into registers, applies the twiddles and a radix-R
+// butterfly, and writes the R results back in natural order.  Radices are
+// 8, 4, 2 and 3; TPP threads share one pencil.  Unnormalised; sign -1 is the
+// forward transform exp(-2 pi i k x / N) (G.c:209-273).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pfdd {
+
+template <typename T> struct V2;
+template <> struct V2<double> { using t = double2; };
+template <> struct V2<float> { using t = float2; };
+template <typename T> using cx = typename V2<T>::t;
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  cx<T> r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename T> __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+// (s i) * a for s = +-1
+template <typename T> __device__ __forceinline__ cx<T> mul_si(cx<T> a, T s) { return {-s * a.y, s * a.x}; }
+
+// ---------------------------------------------------------------- plans
+struct Plan {
+  int np;      // number of passes
+  int r[12];   // radix of each pass
+  int tpp;     // threads per pencil = N / max radix
+};
+
+constexpr Plan make_plan(int n) {
+  Plan p{};
+  int m = n, rmax = 1;
+  while (m % 8 == 0) { p.r[p.np++] = 8; m /= 8; }
+  if (m % 4 == 0) { p.r[p.np++] = 4; m /= 4; }
+  if (m % 2 == 0) { p.r[p.np++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.r[p.np++] = 3; m /= 3; }
+  for (int i = 0; i < p.np; i++) rmax = p.r[i] > rmax ? p.r[i] : rmax;
+  p.tpp = n / rmax;
+  return p;
+}
+// product of the radices before pass k (the Stockham "Ns")
+constexpr int ns_before(int n, int k) {
+  Plan p = make_plan(n);
+  int s = 1;
+  for (int i = 0; i < k; i++) s *= p.r[i];
+  return s;
+}
+constexpr bool supported_len(int n) {
+  if (n < 2) return false;
+  int m = n;
+  while (m % 2 == 0) m /= 2;
+  while (m % 3 == 0) m /= 3;
+  return m == 1;
+}
+
+// ---------------------------------------------------------------- butterflies
+template <typename T, int R> struct Bfly;
+
+template <typename T> struct Bfly<T, 2> {
+  __device__ __forceinline__ static void run(cx<T>* v, T) {
+    cx<T> a = v[0];
+    v[0] = cadd<T>(a, v[1]);
+    v[1] = csub<T>(a, v[1]);
+  }
+};
+
+template <typename T> struct Bfly<T, 4> {
+  __device__ __forceinline__ static void run(cx<T>* v, T s) {
+    cx<T> t0 = cadd<T>(v[0], v[2]), t1 = csub<T>(v[0], v[2]);
+    cx<T> t2 = cadd<T>(v[1], v[3]), t3 = mul_si<T>(csub<T>(v[1], v[3]), s);
+    v[0] = cadd<T>(t0, t2);
+    v[2] = csub<T>(t0, t2);
+    v[1] = cadd<T>(t1, t3);
+    v[3] = csub<T>(t1, t3);
+  }
+};
+
+template <typename T> struct Bfly<T, 3> {
+  __device__ __forceinline__ static void run(cx<T>* v, T s) {
+    const T h = (T)0.86602540378443864676;  // sqrt(3)/2
+    cx<T> t = cadd<T>(v[1], v[2]), u = csub<T>(v[1], v[2]);
+    cx<T> m = {v[0].x - (T)0.5 * t.x, v[0].y - (T)0.5 * t.y};
+    cx<T> hu = mul_si<T>({h * u.x, h * u.y}, s);
+    v[0] = cadd<T>(v[0], t);
+    v[1] = cadd<T>(m, hu);
+    v[2] = csub<T>(m, hu);
+  }
+};
+
+// 8 = 2 x 4: two radix-4 on the even and odd samples, then w8 twiddles.
+template <typename T> struct Bfly<T, 8> {
+  __device__ __forceinline__ static void run(cx<T>* v, T s) {
+    const T c = (T)0.70710678118654752440;  // sqrt(1/2)
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Bfly<T, 4>::run(e, s);
+    Bfly<T, 4>::run(o, s);
+    // o[q] *= w8^(s q): q=1: c(1 + s i), q=2: s i, q=3: c(-1 + s i)
+    cx<T> o1 = {c * (o[1].x - s * o[1].y), c * (o[1].y + s * o[1].x)};
+    cx<T> o2 = mul_si<T>(o[2], s);
+    cx<T> o3 = {c * (-o[3].x - s * o[3].y), c * (-o[3].y + s * o[3].x)};
+    v[0] = cadd<T>(e[0], o[0]);
+    v[4] = csub<T>(e[0], o[0]);
+    v[1] = cadd<T>(e[1], o1);
+    v[5] = csub<T>(e[1], o1);
+    v[2] = cadd<T>(e[2], o2);
+    v[6] = csub<T>(e[2], o2);
+    v[3] = cadd<T>(e[3], o3);
+    v[7] = csub<T>(e[3], o3);
+  }
+};
+
+// ---------------------------------------------------------------- passes
+// One Stockham pass on a pencil of N points at sm (LDS).  q in [0, TPP).
+// tw[m] = exp(+2 pi i m / N); the sign flips its imaginary part.
+template <typename T, int N, int R, int NSB, int TPP>
+__device__ __forceinline__ void stockham_pass(cx<T>* sm, int q, T s, const cx<T>* __restrict__ tw) {
+  constexpr int M = N / R;
+  constexpr int J = (M + TPP - 1) / TPP;
+  constexpr int STEP = N / (NSB * R);
+  cx<T> v[J][R];
+#pragma unroll
+  for (int jj = 0; jj < J; ++jj) {
+    const int j = q + jj * TPP;
+    if ((M % TPP == 0) || j < M) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[jj][r] = sm[j + r * M];
+      if (NSB > 1) {
+        const int k = j % NSB;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[k * r * STEP];
+          w.y *= s;
+          v[jj][r] = cmul<T>(v[jj][r], w);
+        }
+      }
+      Bfly<T, R>::run(v[jj], s);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int jj = 0; jj < J; ++jj) {
+    const int j = q + jj * TPP;
+    if ((M % TPP == 0) || j < M) {
+      const int d = (j / NSB) * NSB * R + (j % NSB);
+#pragma unroll
+      for (int r = 0; r < R; ++r) sm[d + r * NSB] = v[jj][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int K>
+__device__ __forceinline__ void fft_lds_from(cx<T>* sm, int q, T s, const cx<T>* __restrict__ tw) {
+  constexpr Plan pl = make_plan(N);
+  if constexpr (K < pl.np) {
+    stockham_pass<T, N, pl.r[K], ns_before(N, K), pl.tpp>(sm, q, s, tw);
+    fft_lds_from<T, N, K + 1>(sm, q, s, tw);
+  }
+}
+
+// Full N-point transform of the pencil at sm; every thread of the block must
+// call it (it synchronises).  The caller synchronises after filling sm.
+template <typename T, int N>
+__device__ __forceinline__ void fft_lds(cx<T>* sm, int q, T s, const cx<T>* __restrict__ tw) {
+  fft_lds_from<T, N, 0>(sm, q, s, tw);
+}
+
+// LDS row pitch (in complex elements) for a pencil of N points.
+template <int N> constexpr int pitch() { return N + 1; }
+
+}  // namespace pfdd

@@ -1,0 +1,457 @@
+// pfdd_kernels.hpp -- HIP kernels of the GL step (gfx950).
+//
+// Data layout in HBM (one context):
+//   xi    double2 [S][N1][N2][N3]   FP64 master order parameter (complex, as G.c:64)
+//   W     cx<T>   [S][N1][N2][N3]   FFT work array: data / data2 of the reference
+//   Bsym  T       [S(S+1)/2][P]     upper triangle of B(k), stored mode only
+//   mask  u8      [S][P]            1 = frozen (xi_bc != 0), only when some point is frozen
+//   taupp double  [P][S]            per-point resolved shear, only when not uniform
+// T = double (PFDD_FP64) or float (PFDD_MIXED).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pfdd_fft.hpp"
+
+namespace pfdd {
+
+constexpr int MAXS = 16;
+
+// Constants of the elastic kernel, shared by every k-point (read through the
+// scalar cache: every lane of a wave reads the same address).
+struct ElasticConsts {
+  double mu, ll, mup, inv_mu;      // C44, C12, C11-C12-2C44 (Bmatrix G.c:345-347)
+  double sig[MAXS][9];             // s_a = C:e_a
+  double SE[MAXS][MAXS];           // s_a:e_b
+  double Tsum[MAXS];               // sum_ij s_a,ij
+  double T[9];                     // T_pq = sum_ij C_pqij
+  double C[81];                    // C_ijkl
+  double eps[MAXS][9];             // e_a
+  double d[3];                     // grid spacing
+  int n[3];
+  int ns;
+};
+
+// frec, G.c:2698-2741
+__device__ __forceinline__ double frec1(int i, int n, double d) {
+  if (i == 0) return 0.0;
+  if (i < n / 2) return (double)i / (double)n / d;
+  return ((double)i - (double)n) / (double)n / d;
+}
+
+// Green's function of Bmatrix (G.c:386) as a symmetric 3x3 (g00 g01 g02 g11 g12 g22).
+// Returns false at k = 0.
+__device__ __forceinline__ bool green6(const double f[3], const ElasticConsts* __restrict__ ec, double g[6]) {
+  const double fk2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+  if (!(fk2 > 0.0)) return false;
+  const double mu = ec->mu, ll = ec->ll, mup = ec->mup;
+  const double z0 = f[0] * f[0] / fk2, z1 = f[1] * f[1] / fk2, z2 = f[2] * f[2] / fk2;
+  const double m0 = mu + mup * z0, m1 = mu + mup * z1, m2 = mu + mup * z2;
+  const double kk = 1.0 + (mu + ll) * (1 / m0 * z0 + 1 / m1 * z1 + 1 / m2 * z2);
+  const double inv = 1 / fk2, c = (mu + ll) / kk;
+  const double m[3] = {m0, m1, m2};
+  int t = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = i; k < 3; k++)
+      g[t++] = inv * ((i == k ? 1.0 : 0.0) / m[k] - f[i] * f[k] / fk2 / m[i] / m[k] * c);
+  return true;
+}
+
+__device__ __forceinline__ void gmul(const double g[6], const double u[3], double o[3]) {
+  o[0] = g[0] * u[0] + g[1] * u[1] + g[2] * u[2];
+  o[1] = g[1] * u[0] + g[3] * u[1] + g[4] * u[2];
+  o[2] = g[2] * u[0] + g[4] * u[1] + g[5] * u[2];
+}
+
+__device__ __forceinline__ void kvec(long pt, const ElasticConsts* __restrict__ ec, double f[3]) {
+  const int n2 = ec->n[1], n3 = ec->n[2];
+  const int k = (int)(pt % n3);
+  const long r = pt / n3;
+  const int j = (int)(r % n2);
+  const int i = (int)(r / n2);
+  f[0] = frec1(i, ec->n[0], ec->d[0]);
+  f[1] = frec1(j, n2, ec->d[1]);
+  f[2] = frec1(k, n3, ec->d[2]);
+}
+
+__device__ __forceinline__ int tri_index(int a, int b, int S) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo * S - lo * (lo - 1) / 2 + (hi - lo);
+}
+
+// ---------------------------------------------------------------- reductions
+// Block sum (blockDim.x a multiple of 64, at most 1024).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < nw; i++) s += red[i];
+  return s;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------- FFT kernels
+template <typename T, int N> constexpr int tile_rows() {
+  constexpr int tpp = make_plan(N).tpp;
+  int t = 256 / tpp;
+  if (t < 1) t = 1;
+  const int lim = 32768 / (pitch<N>() * (int)sizeof(cx<T>));
+  while (t > lim && t > 1) t /= 2;
+  return t;
+}
+template <typename T, int N> constexpr int tile_cols() {
+  constexpr int tpp = make_plan(N).tpp;
+  int t = 256 / tpp;
+  if (t < 16) t = 16;
+  const int lim = 65536 / (pitch<N>() * (int)sizeof(cx<T>));
+  while (t > lim && t > 1) t /= 2;
+  while (t * tpp > 1024 && t > 1) t /= 2;
+  return t;
+}
+
+// Transform along the contiguous axis: rows of N consecutive elements.
+template <typename T, int N>
+__global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
+k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict__ tw) {
+  constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), NT = TILE * TPP, LD = pitch<N>();
+  __shared__ cx<T> sm[TILE * LD];
+  const long row0 = (long)blockIdx.x * TILE;
+  const int nr = (int)((nrows - row0) < TILE ? (nrows - row0) : TILE);
+  cx<T>* base = data + row0 * N;
+  for (int e = threadIdx.x; e < nr * N; e += NT) sm[(e / N) * LD + (e % N)] = base[e];
+  __syncthreads();
+  const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
+  fft_lds<T, N>(sm + p * LD, q, sign, tw);
+  for (int e = threadIdx.x; e < nr * N; e += NT) base[e] = sm[(e / N) * LD + (e % N)];
+}
+
+// Transform along a strided axis of an [outer][N][inner] array; each block
+// takes `tile` consecutive inner indices (tile divides inner, tile <= TILE).
+template <typename T, int N>
+__global__ void __launch_bounds__((tile_cols<T, N>() * make_plan(N).tpp))
+k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* __restrict__ tw) {
+  constexpr int TPP = make_plan(N).tpp, TILE = tile_cols<T, N>(), NT = TILE * TPP, LD = pitch<N>();
+  __shared__ cx<T> sm[TILE * LD];
+  const long tiles_per_outer = inner / tile;
+  const long o = blockIdx.x / tiles_per_outer;
+  const long i0 = (blockIdx.x % tiles_per_outer) * tile;
+  cx<T>* base = data + o * N * inner + i0;
+  for (int e = threadIdx.x; e < N * tile; e += NT) {
+    const int n = e / tile, p = e % tile;
+    sm[p * LD + n] = base[(long)n * inner + p];
+  }
+  __syncthreads();
+  const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
+  fft_lds<T, N>(sm + p * LD, q, sign, tw);  // pencils p >= tile transform garbage, never stored
+  for (int e = threadIdx.x; e < N * tile; e += NT) {
+    const int n = e / tile, p2 = e % tile;
+    base[(long)n * inner + p2] = sm[p2 * LD + n];
+  }
+}
+
+// ---------------------------------------------------------------- GL kernels
+struct StepArgs {
+  long P;
+  int ns;
+  int nblk;             // blocks per field of k_update
+  double invP_unused;
+  double uq2, pi;
+  double cd2[MAXS], asf2[MAXS], dslip[MAXS], tau[MAXS];
+};
+
+constexpr int UPD_THREADS = 256, UPD_EPT = 4;
+
+// Last-block-done finish of a per-field sum: partials[f][nblk] -> out[f]
+// (and out[ns] = sum over fields).  The counter resets itself.
+__device__ __forceinline__ void finish_field_sums(const double* partials, int ns, int nblk,
+                                                  unsigned* counter, double* out, int* step_ctr,
+                                                  double* red) {
+  __shared__ bool last;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    const unsigned total = (unsigned)(gridDim.x * gridDim.y);
+    last = (atomicAdd(counter, 1u) == total - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double* vp = partials;
+  double* dst = out;
+  if (step_ctr) dst = out + (long)(*step_ctr) * (ns + 1);
+  double tot = 0.0;
+  for (int f = 0; f < ns; f++) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) v += vp[(long)f * nblk + b];
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) {
+      dst[f] = s;
+      tot += s;
+    }
+  }
+  if (threadIdx.x == 0) {
+    dst[ns] = tot;
+    if (step_ctr) *step_ctr = *step_ctr + 1;
+    *counter = 0u;
+  }
+}
+
+// plasticevolv (G.c:3470-3501) fused with the copy xi -> data (G.c:4581-4590)
+// and the gamma sums.  grid = (nblk, S).
+template <typename T, bool MASK, bool TAUPP>
+__global__ void __launch_bounds__(UPD_THREADS)
+k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
+         const double* __restrict__ taupp, StepArgs a, double* partials, unsigned* counter,
+         double* hist, int* step_ctr) {
+  __shared__ double red[UPD_THREADS / 64];
+  const int f = blockIdx.y;
+  const long P = a.P;
+  const double nsize = (double)P;
+  const long off = (long)f * P;
+  const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
+  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f], tauf = a.tau[f];
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < UPD_EPT; e++) {
+    const long pt = base + (long)e * UPD_THREADS;
+    if (pt < P) {
+      double2 x = xi[off + pt];
+      const cx<T> d = W[off + pt];
+      if (!MASK || mask[off + pt] == 0) {
+        const double t = TAUPP ? taupp[pt * a.ns + f] : tauf;
+        x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / dsl);
+        x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
+        xi[off + pt] = x;
+      }
+      W[off + pt] = {(T)x.x, (T)x.y};
+      acc += x.x / nsize;
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[(long)f * a.nblk + blockIdx.x] = s;
+  finish_field_sums(partials, a.ns, a.nblk, counter, hist, step_ctr, red);
+}
+
+// Sum of Re xi per field (avestrain's xi means, G.c:670-686).  grid = (nblk, S).
+__global__ void __launch_bounds__(UPD_THREADS)
+k_xi_sums(const double2* __restrict__ xi, long P, int ns, int nblk, double* partials,
+          unsigned* counter, double* out) {
+  __shared__ double red[UPD_THREADS / 64];
+  const int f = blockIdx.y;
+  const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
+  double acc = 0.0;
+  for (int e = 0; e < UPD_EPT; e++) {
+    const long pt = base + (long)e * UPD_THREADS;
+    if (pt < P) acc += xi[(long)f * P + pt].x;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[(long)f * nblk + blockIdx.x] = s;
+  finish_field_sums(partials, ns, nblk, counter, out, nullptr, red);
+}
+
+template <typename T>
+__global__ void k_xi_to_w(const double2* __restrict__ xi, cx<T>* __restrict__ W, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double2 x = xi[i];
+    W[i] = {(T)x.x, (T)x.y};
+  }
+}
+
+__global__ void k_set_int(int* p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Contraction data2_a(k) = sum_b B_ab(k) data_b(k) (G.c:4609-4629), in place
+// on W.  B comes from Bsym (STORED) or is rebuilt per k in FP64 from
+// B_ab = (s_a:e_b - u_a.G.u_b)/mu, u_a = s_a f.  quirk != 0 adds
+// data_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441).
+template <typename T, int SM, bool STORED>
+__global__ void __launch_bounds__(256)
+k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConsts* __restrict__ ec,
+           long P, int quirk) {
+  const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= P) return;
+  const int S = ec->ns;
+  cx<T> v[SM];
+#pragma unroll
+  for (int b = 0; b < SM; b++)
+    if (b < S) v[b] = W[(long)b * P + pt];
+  double f[3], g[6];
+  kvec(pt, ec, f);
+  const bool nz = green6(f, ec, g);
+  double u[SM][3];
+  if (!STORED || quirk) {
+#pragma unroll
+    for (int b = 0; b < SM; b++)
+      if (b < S) {
+        const double* s = ec->sig[b];
+        u[b][0] = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
+        u[b][1] = s[3] * f[0] + s[4] * f[1] + s[5] * f[2];
+        u[b][2] = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
+      }
+  }
+  double tf[3] = {0, 0, 0};
+  if (quirk) {
+    const double* t9 = ec->T;
+    tf[0] = t9[0] * f[0] + t9[1] * f[1] + t9[2] * f[2];
+    tf[1] = t9[3] * f[0] + t9[4] * f[1] + t9[5] * f[2];
+    tf[2] = t9[6] * f[0] + t9[7] * f[1] + t9[8] * f[2];
+  }
+#pragma unroll
+  for (int a = 0; a < SM; a++) {
+    if (a >= S) continue;
+    cx<T> acc = {(T)0, (T)0};
+    if (nz) {
+      double gu[3] = {0, 0, 0};
+      if (!STORED || quirk) gmul(g, u[a], gu);
+#pragma unroll
+      for (int b = 0; b < SM; b++) {
+        if (b >= S) continue;
+        T bab;
+        if (STORED) {
+          bab = Bsym[(long)tri_index(a, b, S) * P + pt];
+        } else {
+          bab = (T)((ec->SE[a][b] - (gu[0] * u[b][0] + gu[1] * u[b][1] + gu[2] * u[b][2])) * ec->inv_mu);
+        }
+        acc.x += v[b].x * bab;
+        acc.y += v[b].y * bab;
+      }
+      if (quirk) {
+        const T dd = (T)((ec->Tsum[a] - (gu[0] * tf[0] + gu[1] * tf[1] + gu[2] * tf[2])) * ec->inv_mu);
+        acc.x += v[a].x * dd;
+        acc.y += v[a].y * dd;
+      }
+    }
+    W[(long)a * P + pt] = acc;
+  }
+}
+
+// Stored B(k): upper triangle in the working precision, built in FP64.
+template <typename T, int SM>
+__global__ void __launch_bounds__(256)
+k_build_B(T* __restrict__ Bsym, const ElasticConsts* __restrict__ ec, long P) {
+  const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= P) return;
+  const int S = ec->ns;
+  double f[3], g[6];
+  kvec(pt, ec, f);
+  const bool nz = green6(f, ec, g);
+  double u[SM][3];
+#pragma unroll
+  for (int b = 0; b < SM; b++)
+    if (b < S) {
+      const double* s = ec->sig[b];
+      u[b][0] = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
+      u[b][1] = s[3] * f[0] + s[4] * f[1] + s[5] * f[2];
+      u[b][2] = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
+    }
+#pragma unroll
+  for (int a = 0; a < SM; a++) {
+    if (a >= S) continue;
+    double gu[3];
+    gmul(g, u[a], gu);
+#pragma unroll
+    for (int b = 0; b < SM; b++) {
+      if (b < a || b >= S) continue;
+      const double bab = nz ? (ec->SE[a][b] - (gu[0] * u[b][0] + gu[1] * u[b][1] + gu[2] * u[b][2])) * ec->inv_mu : 0.0;
+      Bsym[(long)tri_index(a, b, S) * P + pt] = (T)bab;
+    }
+  }
+}
+
+// Strain spectrum: beta_ij(k) = sum_a FF_a,ij xi_a = f_i (G v)_j with
+// v = sum_a u_a xi_a (Fmatrix G.c:565-571, strain G.c:768-781); only its
+// symmetric part is kept (strain G.c:806-817): E[t], t over (00,01,02,11,12,22).
+template <typename T, int SM>
+__global__ void __launch_bounds__(256)
+k_strain_spectrum(const cx<T>* __restrict__ W, cx<T>* __restrict__ E, const ElasticConsts* __restrict__ ec, long P) {
+  const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= P) return;
+  const int S = ec->ns;
+  double f[3], g[6];
+  kvec(pt, ec, f);
+  const bool nz = green6(f, ec, g);
+  double vr[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+  if (nz) {
+#pragma unroll
+    for (int b = 0; b < SM; b++) {
+      if (b >= S) continue;
+      const double* s = ec->sig[b];
+      const double u0 = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
+      const double u1 = s[3] * f[0] + s[4] * f[1] + s[5] * f[2];
+      const double u2 = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
+      const cx<T> x = W[(long)b * P + pt];
+      vr[0] += u0 * (double)x.x; vr[1] += u1 * (double)x.x; vr[2] += u2 * (double)x.x;
+      vi[0] += u0 * (double)x.y; vi[1] += u1 * (double)x.y; vi[2] += u2 * (double)x.y;
+    }
+  }
+  double gr[3], gi[3];
+  gmul(g, vr, gr);
+  gmul(g, vi, gi);
+  const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    const int i = ii[t], j = jj[t];
+    cx<T> o = {(T)0, (T)0};
+    if (nz) {
+      o.x = (T)((f[i] * gr[j] + f[j] * gr[i]) / 2.);
+      o.y = (T)((f[i] * gi[j] + f[j] * gi[i]) / 2.);
+    }
+    E[(long)t * P + pt] = o;
+  }
+}
+
+// Per point: eps = Re E / P + sym(avepst); eps* = sum_a e_a Re xi_a;
+// sigma = C:(eps - eps*) (stress G.c:897-919); 18 block sums -> partials.
+template <typename T, int SM>
+__global__ void __launch_bounds__(256)
+k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, const ElasticConsts* __restrict__ ec,
+                const double* __restrict__ avepst, long P, double* partials, int nblk, unsigned* counter,
+                double* out, double* eps_field, double* sig_field) {
+  __shared__ double red[4];
+  const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int S = ec->ns;
+  double e[9] = {0}, sg[9] = {0};
+  if (pt < P) {
+    const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      const int i = ii[t], j = jj[t];
+      const double v = (double)E[(long)t * P + pt].x / (double)P + (avepst[3 * i + j] + avepst[3 * j + i]) / 2.;
+      e[3 * i + j] = v;
+      e[3 * j + i] = v;
+    }
+    double es[9] = {0};
+#pragma unroll
+    for (int a = 0; a < SM; a++) {
+      if (a >= S) continue;
+      const double x = xi[(long)a * P + pt].x;
+#pragma unroll
+      for (int kl = 0; kl < 9; kl++) es[kl] += ec->eps[a][kl] * x;
+    }
+#pragma unroll
+    for (int ij = 0; ij < 9; ij++) {
+      double v = 0.0;
+#pragma unroll
+      for (int kl = 0; kl < 9; kl++) v += ec->C[ij * 9 + kl] * (e[kl] - es[kl]);
+      sg[ij] = v;
+    }
+    if (eps_field)
+      for (int ij = 0; ij < 9; ij++) eps_field[(long)ij * P + pt] = e[ij];
+    if (sig_field)
+      for (int ij = 0; ij < 9; ij++) sig_field[(long)ij * P + pt] = sg[ij];
+  }
+  for (int c = 0; c < 18; c++) {
+    const double s = block_sum(c < 9 ? e[c] : sg[c - 9], red);
+    if (threadIdx.x == 0) partials[(long)c * nblk + blockIdx.x] = s;
+  }
+  finish_field_sums(partials, 18, nblk, counter, out, nullptr, red);
+}
+
+}  // namespace pfdd

@@ -1,0 +1,320 @@
+"""GPU parity: the HIP path (through the C ABI of libpfdd_gpu.so) against the
+FP64 oracle, the committed golden trajectories and the compiled reference."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oraclepy as O
+import pfdd_amd as pa
+import physics as ph
+import refpy as R
+from pfdd_amd import configs as C
+from pfdd_amd import hip
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if hip.device_count() < 1:
+        pytest.skip("no GPU visible")
+
+
+def run_gpu(w, steps, precision=pa.FP64, bmode=pa.B_RECOMPUTE, quirk=True, xi_bc=None, tau=None,
+            tau_pp=False):
+    with pa.Context(w.params(precision, bmode)) as c:
+        c.set_tau(w.tau if tau is None else tau, per_point=tau_pp)
+        c.upload_xi(w.xi, xi_bc)
+        c.begin_outer(quirk)
+        g, ga = c.gl_steps(steps)
+        return c.download_xi(), g, ga
+
+
+def run_oracle(w, steps, quirk=True, xi_bc=None, tau=None, tau_pp=False):
+    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * w.ns, C.asf2_reference(w.ns), [1.0] * w.ns)
+    o = O.Oracle(p)
+    o.set_xi(w.xi, xi_bc)
+    o.set_tau(w.tau if tau is None else tau, per_point=tau_pp)
+    o.prologue(quirk)
+    g, ga = o.gl_steps(steps)
+    return o.get_xi(), g, ga
+
+
+# ---------------------------------------------------------------- FFT (row a1)
+FFT_SHAPES = [(32, 2, 32), (16, 16, 16), (8, 12, 6), (48, 48, 48), (96, 32, 16), (128, 128, 128),
+              (3, 4, 2), (256, 64, 4), (512, 8, 2), (24, 384, 2), (64, 192, 12)]
+
+
+@pytest.mark.parametrize("shape", FFT_SHAPES)
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+def test_fft_matches_oracle(shape, prec):
+    say(f"fft {shape} prec={prec}")
+    rng = np.random.default_rng(sum(shape))
+    batch = 2
+    x = rng.standard_normal((batch,) + shape) + 1j * rng.standard_normal((batch,) + shape)
+    eps = ph.fcc_eps([0])
+    w = C.Workload("fft", shape, eps, np.zeros((1, 3)), np.zeros((1,) + shape, complex), np.zeros(1),
+                   np.zeros((3, 3)), 0)
+    dt = np.complex128 if prec == pa.FP64 else np.complex64
+    tol = 1e-13 if prec == pa.FP64 else 3e-6
+    with pa.Context(w.params(prec)) as c:
+        for sign in (pa.FORWARD, pa.BACKWARD):
+            buf = hip.DeviceBuffer.from_array(x.astype(dt))
+            c.fft(buf.ptr, batch, sign, prec)
+            c.sync()
+            got = buf.to_array(dt, x.shape)
+            want = O.fft3d(x.astype(dt).astype(np.complex128), sign)
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < tol, (shape, sign, err)
+
+
+def test_fft_round_trip_256_cubed():
+    say("fft round trip 256^3 fp32")
+    shape = (256, 256, 256)
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    w = C.Workload("fft", shape, ph.fcc_eps([0]), np.zeros((1, 3)), np.zeros((1, 2, 2, 2), complex), np.zeros(1),
+                   np.zeros((3, 3)), 0)
+    with pa.Context(w.params(pa.MIXED)) as c:
+        buf = hip.DeviceBuffer.from_array(x)
+        c.fft(buf.ptr, 1, pa.FORWARD, pa.MIXED)
+        c.sync()
+        X = buf.to_array(np.complex64, shape)
+        assert abs(X[0, 0, 0] - x.astype(np.complex128).sum()) < 1e-3 * abs(X).max()
+        c.fft(buf.ptr, 1, pa.BACKWARD, pa.MIXED)
+        c.sync()
+        y = buf.to_array(np.complex64, shape) / np.prod(shape)
+        assert np.abs(y - x).max() < 1e-5
+
+
+# ---------------------------------------------------------------- GL loop (rows a3-a10)
+def test_c0_full_run_matches_golden():
+    """c0: 32^3 x 3, 500 GL steps, FP64: xi to 1e-10 against the oracle trajectory."""
+    say("c0 500 steps fp64 vs gl_c0.npz")
+    g0 = np.load(os.path.join(GOLD, "gl_c0.npz"))
+    w = C.c0()
+    xi, g, ga = run_gpu(w, int(g0["steps"]))
+    assert np.abs(xi.real - g0["re"]).max() <= 1e-10
+    assert np.abs(xi.imag).max() <= 1e-10
+    assert np.abs(g - g0["gamma"]).max() <= 1e-10 * max(1.0, np.abs(g0["gamma"]).max())
+    assert np.abs(ga - g0["gamma_a"]).max() <= 1e-10
+
+
+@pytest.mark.parametrize("bmode", [pa.B_RECOMPUTE, pa.B_STORED])
+def test_c0_live_oracle_both_bmodes(bmode):
+    say(f"c0 60 steps bmode={bmode} vs live oracle")
+    w = C.c0()
+    xo, go, gao = run_oracle(w, 60)
+    xg, gg, gag = run_gpu(w, 60, bmode=bmode)
+    assert np.abs(xg - xo).max() <= 1e-12
+    assert np.abs(gag - gao).max() <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (48, 24, 12), (8, 96, 6)])
+def test_fcc12_small_grids_vs_oracle(shape):
+    say(f"fcc NS=12 {shape} 25 steps vs oracle")
+    w = C.small_fcc(shape)
+    xo, go, gao = run_oracle(w, 25)
+    xg, gg, gag = run_gpu(w, 25)
+    assert np.abs(xg - xo).max() <= 1e-12
+    assert np.abs(gg - go).max() <= 1e-12 * max(1.0, np.abs(go).max())
+
+
+@pytest.mark.parametrize("quirk", [True, False])
+def test_prologue_quirk_flag(quirk):
+    say(f"quirk={quirk}")
+    w = C.small_fcc((16, 16, 16), ns=3)
+    xo, _, _ = run_oracle(w, 5, quirk=quirk)
+    xg, _, _ = run_gpu(w, 5, quirk=quirk)
+    assert np.abs(xg - xo).max() <= 1e-12
+    xother, _, _ = run_oracle(w, 5, quirk=not quirk)
+    assert np.abs(xg - xother).max() > 1e-6
+
+
+@pytest.mark.skipif(not R.available(), reason="compiled reference oracle/_ref not built")
+def test_reference_grid_vs_reference_functions():
+    """32x2x32, NS=2: GPU FP64 vs the GL loop run through the reference's own
+    compiled functions (FP32 arrays) and vs the oracle."""
+    say("reference grid 32x2x32 NS=2 vs compiled reference")
+    eps = ph.fcc_eps([0, 1], ph.theta_nz())
+    rng = np.random.default_rng(5)
+    xi0 = np.zeros((2, R.P), dtype=np.complex128)
+    X = np.zeros((32, 2, 32))
+    X[8:20, :, 10:22] = 1.0
+    xi0[0] = X.reshape(-1)
+    xi0[1] = 0.3 * rng.random(R.P)
+    tau = np.array([0.01, -0.004])
+    xr, gr = R.gl_loop(xi0, eps, R.epsv_basis(), ph.C11, ph.C12, ph.C44, [0.1, 0.1], ph.asf2_reference(2),
+                       [1.0, 1.0], tau, 10)
+    w = C.Workload("ref", (32, 2, 32), eps, np.zeros((2, 3)), xi0.reshape(2, 32, 2, 32), tau, np.zeros((3, 3)), 10)
+    xg, gg, _ = run_gpu(w, 10)
+    xo, go, _ = run_oracle(w, 10)
+    assert np.abs(xg.reshape(2, -1) - xr).max() < 1e-6
+    assert np.abs(xg - xo).max() < 1e-12
+    assert np.abs(gg - gr).max() < 1e-8
+
+
+def test_c1_first_steps_vs_oracle():
+    """c1 (128^3 x 12): the first 3 GL steps against the live oracle."""
+    say("c1 3 steps vs live oracle")
+    w = C.c1()
+    xo, go, gao = run_oracle(w, 3)
+    xg, gg, gag = run_gpu(w, 3)
+    assert np.abs(xg - xo).max() <= 1e-12
+    assert np.abs(gag - gao).max() <= 1e-12
+
+
+def test_c1_full_run_matches_golden():
+    """c1: 1000 GL steps FP64 against the committed oracle trajectory, 1e-10."""
+    path = os.path.join(GOLD, "gl_c1.npz")
+    if not os.path.exists(path):
+        pytest.skip("gl_c1.npz not generated")
+    say("c1 1000 steps fp64 vs gl_c1.npz")
+    g1 = np.load(path)
+    w = C.c1()
+    xi, g, ga = run_gpu(w, int(g1["steps"]))
+    re = xi.real
+    idx = g1["i_idx"]
+    assert np.abs(re[:, idx] - g1["planes_i"]).max() <= 1e-10
+    assert np.abs(re[:, :, 5, :] - g1["plane_j5"]).max() <= 1e-10
+    assert np.abs(re[:, :, :, 77] - g1["plane_k77"]).max() <= 1e-10
+    assert np.abs(xi.imag).max() <= 1e-10
+    assert np.abs(ga - g1["gamma_a"]).max() <= 1e-10
+    assert np.allclose(re.reshape(w.ns, -1).sum(1), g1["sums"], rtol=1e-11, atol=1e-8)
+
+
+def test_c1_stored_equals_recompute():
+    say("c1 stored vs recompute 20 steps")
+    w = C.c1()
+    xa, _, _ = run_gpu(w, 20, bmode=pa.B_RECOMPUTE)
+    xb, _, _ = run_gpu(w, 20, bmode=pa.B_STORED)
+    assert np.abs(xa - xb).max() <= 1e-13
+
+
+def test_uniform_xi_closed_form_at_128_cubed():
+    """Size-independent: uniform xi only excites k=0 where B=0, so every step
+    is the scalar recurrence xi <- xi - CD2 (Asf2 pi sin(2 pi xi) - tau)."""
+    say("uniform xi 128^3 x 12 closed form")
+    w = C.c1()
+    x0 = np.linspace(0.0, 0.9, w.ns)
+    w.xi = np.ascontiguousarray(np.ones((w.ns,) + w.shape, dtype=np.complex128) * x0[:, None, None, None])
+    xg, gg, gag = run_gpu(w, 40)
+    x = x0.copy()
+    asf = np.array(C.asf2_reference(w.ns))
+    for s in range(40):
+        x = x - 0.1 * (asf * C.PI_REF * np.sin(2.0 * C.PI_REF * x) - w.tau)
+        assert np.abs(gag[s] - x).max() < 1e-12
+    assert np.abs(xg.real - x[:, None, None, None]).max() < 1e-12
+
+
+def test_frozen_points_and_per_point_tau():
+    say("xi_bc mask + per-point tau")
+    w = C.small_fcc((16, 16, 16), ns=4)
+    rng = np.random.default_rng(3)
+    bc = np.zeros((w.ns,) + w.shape, dtype=np.complex128)
+    bc[1, 3:7] = 1.0
+    bc[2, :, :, ::5] = 1.0
+    tau = rng.uniform(-0.02, 0.02, (w.P, w.ns))
+    xo, go, _ = run_oracle(w, 15, xi_bc=bc, tau=tau, tau_pp=True)
+    xg, gg, _ = run_gpu(w, 15, xi_bc=bc, tau=tau, tau_pp=True)
+    assert np.abs(xg - xo).max() <= 1e-12
+    assert np.array_equal(xg[1, 3:7], w.xi[1, 3:7])
+
+
+def test_graph_chunks_equal_single_steps():
+    say("graph replay vs eager single steps")
+    w = C.small_fcc((16, 16, 16), ns=3)
+    with pa.Context(w.params()) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi)
+        c.begin_outer(True)
+        g1, _ = c.gl_steps(37)
+        x1 = c.download_xi()
+    with pa.Context(w.params()) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi)
+        c.begin_outer(True)
+        gs = [c.gl_steps(1)[0][0] for _ in range(37)]
+        x2 = c.download_xi()
+    assert np.array_equal(x1, x2)
+    assert np.array_equal(g1, np.array(gs))
+
+
+def test_mixed_precision_error_c1():
+    """c1 in mixed precision for 1000 steps against FP64: xi <= 1e-4 and
+    slipped area (sum_a gamma_a P) <= 1e-3 relative."""
+    say("c1 mixed 1000 steps vs fp64")
+    w = C.c1()
+    x64, g64, ga64 = run_gpu(w, 1000)
+    x32, g32, ga32 = run_gpu(w, 1000, precision=pa.MIXED)
+    err_xi = np.abs(x32 - x64).max()
+    area64 = ga64[-1].sum() * w.P
+    area32 = ga32[-1].sum() * w.P
+    say(f"mixed: max|dxi| = {err_xi:.3e}, slipped area rel = {abs(area32 - area64) / abs(area64):.3e}")
+    assert err_xi <= 1e-4
+    assert abs(area32 - area64) <= 1e-3 * abs(area64)
+
+
+# ---------------------------------------------------------------- strain / stress (row a11)
+def test_strain_stress_vs_oracle():
+    say("strain/stress vs oracle")
+    w = C.small_fcc((16, 16, 16), ns=3)
+    ap = w.s_tensor_avepst()
+    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * 3, C.asf2_reference(3), [1.0] * 3)
+    o = O.Oracle(p)
+    o.set_xi(w.xi)
+    ae_o, as_o, ef_o, sf_o = o.strain_stress(ap, fields=True)
+    with pa.Context(w.params()) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi)
+        ae, asg, ef, sf = c.strain_stress(ap, fields=True)
+        with pytest.raises(pa.PfddError) as e:
+            c.gl_steps(1)
+        assert e.value.rc == pa.E_STATE
+    assert np.abs(ae - ae_o).max() <= 1e-13
+    assert np.abs(asg - as_o).max() <= 1e-13
+    assert np.abs(asg - w.sigma * C.MU).max() <= 1e-13
+    assert np.abs(ef - ef_o).max() <= 1e-12
+    assert np.abs(sf - sf_o).max() <= 1e-12
+
+
+# ---------------------------------------------------------------- boundary error behaviour
+def test_error_codes():
+    say("error codes")
+    w = C.small_fcc((8, 8, 8), ns=2)
+    with pa.Context(w.params()) as c:
+        with pytest.raises(pa.PfddError) as e:
+            c.begin_outer()
+        assert e.value.rc == pa.E_STATE
+        bad = w.xi.copy()
+        bad[0, 1, 2, 3] = np.nan
+        with pytest.raises(pa.PfddError) as e:
+            c.upload_xi(bad)
+        assert e.value.rc == pa.E_NAN
+        virt = np.zeros((11,) + w.shape, dtype=np.complex128)
+        virt[:2] = w.xi
+        virt[5, 0, 0, 0] = 0.25
+        with pytest.raises(pa.PfddError) as e:
+            c.upload_xi(virt)
+        assert e.value.rc == pa.E_UNSUPPORTED
+        virt[5, 0, 0, 0] = 0.0
+        c.upload_xi(virt)
+        with pytest.raises(pa.PfddError) as e:
+            c.gl_steps(3)
+        assert e.value.rc == pa.E_STATE
+        c.set_tau(w.tau)
+        c.begin_outer()
+        c.gl_steps(3)
+        out = c.download_xi(11)
+        assert np.all(out[2:] == 0)
+        with pytest.raises(pa.PfddError) as e:
+            c.set_tau(np.array([np.inf, 0.0]))
+        assert e.value.rc == pa.E_NAN

@@ -1,0 +1,130 @@
+"""CPU-side checks of libpfdd_gpu.so: it loads, exports every entry point the
+header declares, its host setup helpers match the compiled reference, and its
+argument validation returns the documented codes (no GPU work is issued)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pfdd_amd as pa
+import physics as ph
+import refpy as R
+from pfdd_amd import configs as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "pfdd_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pfdd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_function():
+    L = pa.load()
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(pa.EXPORTED) == names
+
+
+def test_strerror_names_codes():
+    for rc in (pa.OK, pa.E_HIP, pa.E_ALLOC, pa.E_SHAPE, pa.E_RADIX, pa.E_NAN, pa.E_UNSUPPORTED, pa.E_STATE):
+        assert pa.strerror(rc) and pa.strerror(rc) != "unknown error"
+
+
+def _create_rc(p):
+    h = ctypes.c_void_p()
+    rc = pa.load().pfdd_create(ctypes.byref(p), ctypes.byref(h))
+    assert not h.value
+    return rc
+
+
+def test_create_validates_before_touching_the_device():
+    eps = ph.fcc_eps([0, 1, 2])
+    base = dict(shape=(32, 32, 32), eps=eps, C11=ph.C11, C12=ph.C12, C44=ph.C44, cd2=[0.1] * 3,
+                asf2=ph.asf2_reference(3), dslip=[1.0] * 3)
+    assert _create_rc(pa.make_params(**{**base, "shape": (7, 32, 32)})) == pa.E_RADIX
+    assert _create_rc(pa.make_params(**{**base, "shape": (32, 1, 32)})) == pa.E_RADIX
+    assert _create_rc(pa.make_params(**{**base, "shape": (32, 32, 1024)})) == pa.E_RADIX
+    p = pa.make_params(**base)
+    p.ns = 0
+    assert _create_rc(p) == pa.E_SHAPE
+    p = pa.make_params(**base)
+    p.ns = 17
+    assert _create_rc(p) == pa.E_SHAPE
+    assert _create_rc(pa.make_params(**{**base, "C11": float("nan")})) == pa.E_NAN
+    assert _create_rc(pa.make_params(**{**base, "dslip": [1.0, 0.0, 1.0]})) == pa.E_NAN
+    p = pa.make_params(**base)
+    p.precision = 7
+    assert _create_rc(p) == pa.E_SHAPE
+
+
+def test_splitmix64_known_answers():
+    """First outputs of SplitMix64 for seed 1234567 (Vigna's reference implementation)."""
+    r = C.Rng(1234567)
+    assert [r.next64() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                             4593380528125082431, 16408922859458223821]
+    u = C.Rng(1).uniform()
+    assert 0.0 <= u < 1.0
+
+
+def test_setfcc_geometry():
+    xn, xb = C.setfcc()
+    s3 = float(np.float32(np.sqrt(3)))
+    assert np.all(np.abs(np.abs(xn) - 1 / s3) == 0)          # float sqrt3 as G.c:1820
+    assert np.abs(np.einsum("ai,ai->a", xn, xb)).max() < 1e-15
+    assert np.allclose(np.linalg.norm(xb, axis=1), 1 / np.sqrt(2))
+    assert len({tuple(r) for r in np.round(np.hstack([xn, xb]), 6)}) == 12
+    assert np.array_equal(C.seteps(xn, xb, [1.0] * 12, np.eye(3)), ph.fcc_eps(range(12)))
+
+
+@pytest.mark.skipif(not R.available(), reason="compiled reference oracle/_ref not built")
+def test_setorient_seteps_resolve_match_reference():
+    th_ref = R.setorient()
+    rows = [[np.sqrt(3.) / 3., np.sqrt(3.) / 3., np.sqrt(3.) / 3.], [-np.sqrt(2.) / 2., np.sqrt(2.) / 2., 0.0],
+            [-np.sqrt(6.) / 6., -np.sqrt(6.) / 6., np.sqrt(6.) / 3.]]
+    th = C.setorient(rows)
+    assert np.array_equal(th, th_ref[0])
+    xn, xb = R.set3D1sys()
+    eps_ref, _ = R.seteps(xn, xb, [1.0, 1.0], th_ref)
+    eps = C.seteps(xn, xb, [1.0, 1.0], th)
+    assert np.array_equal(eps, eps_ref)
+    # log.ref:247-254: both eigenstrains are [[0,0,0],[0,0,0],[-1,0,0]] to 6 digits
+    assert np.allclose(eps, [[[0, 0, 0], [0, 0, 0], [-1, 0, 0]]] * 2, atol=5e-7)
+    sig = np.array([[0.003, 0.01, -0.002], [0.01, 0.0, 0.004], [-0.002, 0.004, -0.006]])
+    tau_ref = R.resolSS(np.tile(sig, (R.P, 1, 1)), eps_ref)
+    assert np.array_equal(C.resolve_tau(sig, eps), tau_ref[0])
+    assert np.array_equal(tau_ref[0], tau_ref[-1])
+
+
+def test_loop_voxelisation_closed_forms():
+    # a loop in a grid plane: the Gauss circle count N(8) = 197
+    xi = np.zeros((1, 32, 32, 32), dtype=np.complex128)
+    assert C.add_loop(xi, 0, (0, 0, 1), (16, 16, 16), 8.0) == 197
+    k = np.nonzero(xi[0].real)[2]
+    assert np.all(k == 16)
+    # periodic wrap: the same disc centred on a corner has the same count
+    xi2 = np.zeros((1, 32, 32, 32), dtype=np.complex128)
+    assert C.add_loop(xi2, 0, (0, 0, 1), (0, 0, 0), 8.0) == 197
+    # a (111) loop: every point sits on one lattice plane x+y+z = m
+    xi3 = np.zeros((1, 40, 40, 40), dtype=np.complex128)
+    n = C.add_loop(xi3, 0, (1, 1, 1), (20.2, 19.7, 20.4), 6.0)
+    i, j, kk = np.nonzero(xi3[0].real)
+    assert n == len(i) > 0
+    assert len(set(i + j + kk)) == 1
+    pts = np.stack([i, j, kk], 1) - np.array([20.2, 19.7, 20.4])
+    nh = np.ones(3) / np.sqrt(3)
+    inplane = pts - np.outer(pts @ nh, nh)
+    assert np.linalg.norm(inplane, axis=1).max() <= 6.0
+
+
+def test_workloads_are_reproducible():
+    a, b = C.c0(), C.c0()
+    assert np.array_equal(a.xi, b.xi) and np.array_equal(a.tau, b.tau)
+    assert a.tau[0] == pytest.approx(0.01, abs=1e-15)
+    w = C.small_fcc((16, 16, 16))
+    assert w.ns == 12 and w.xi.real.sum() > 0
