@@ -1,11 +1,15 @@
 """Writes GL-trajectory fixtures computed by the FP64 oracle (oracle/liboracle.so)
 on the seeded workloads of pfdd_amd.configs:
 
-  gl_c0.npz   c0 (32^3, NS=3, 500 GL steps): final Re xi (all points), max |Im xi|,
+  gl_c0.npz   c0 (32^3, NS=3, 500 GL steps): final complex xi (all points),
               gamma and gamma_a for every step.
-  gl_c1.npz   c1 (128^3, NS=12, 1000 GL steps): final Re xi on the planes
+  gl_c1.npz   c1 (128^3, NS=12, 1000 GL steps): final complex xi on the planes
               i = 0, 37, 64, 101 and the slabs j = 5, k = 77, per-field sums and
-              sums of squares of Re xi, gamma and gamma_a for every step.
+              sums of squares of Re xi and Im xi, gamma and gamma_a for every step.
+
+xi is complex on purpose: frec (G.c:2698-2741) puts the Nyquist index at -1/2
+on every axis, so B(k) != B(-k) on planes where only some components are at
+Nyquist, and the reference's C2C loop drives Im xi to O(1e-3).
 
 Run from the repo root (c1 takes tens of minutes on 8 cores):
     python tests/golden/make_gl_golden.py c0 c1
@@ -42,17 +46,18 @@ def main(which):
     if "c0" in which:
         w = C.c0()
         xi, g, ga, dt = run(w, w.steps)
-        np.savez_compressed(os.path.join(HERE, "gl_c0.npz"), re=xi.real, im_max=np.abs(xi.imag).max(),
-                            gamma=g, gamma_a=ga, steps=w.steps, seconds=dt)
+        np.savez_compressed(os.path.join(HERE, "gl_c0.npz"), xi=xi, gamma=g, gamma_a=ga, steps=w.steps,
+                            seconds=dt)
         print("c0", dt, "s")
     if "c1" in which:
         w = C.c1()
         xi, g, ga, dt = run(w, w.steps)
-        re = xi.real
-        np.savez_compressed(os.path.join(HERE, "gl_c1.npz"), planes_i=re[:, C1_PLANES_I], plane_j5=re[:, :, 5, :],
-                            plane_k77=re[:, :, :, 77], i_idx=np.array(C1_PLANES_I),
-                            sums=re.reshape(w.ns, -1).sum(1), sumsq=(re.reshape(w.ns, -1) ** 2).sum(1),
-                            im_max=np.abs(xi.imag).max(), gamma=g, gamma_a=ga, steps=w.steps, seconds=dt)
+        flat = xi.reshape(w.ns, -1)
+        np.savez_compressed(os.path.join(HERE, "gl_c1.npz"), planes_i=xi[:, C1_PLANES_I], plane_j5=xi[:, :, 5, :],
+                            plane_k77=xi[:, :, :, 77], i_idx=np.array(C1_PLANES_I),
+                            sums_re=flat.real.sum(1), sumsq_re=(flat.real ** 2).sum(1),
+                            sums_im=flat.imag.sum(1), sumsq_im=(flat.imag ** 2).sum(1),
+                            gamma=g, gamma_a=ga, steps=w.steps, seconds=dt)
         print("c1", dt, "s")
 
 
