@@ -55,7 +55,6 @@ struct pfdd_ctx {
   double* partials = nullptr;
   size_t partials_len = 0;
   double* hist = nullptr;    // [HIST_CAP][S+1]
-  unsigned* counter = nullptr;
   int* step_ctr = nullptr;
   double* small = nullptr;   // reduction outputs (<= 32 doubles)
   double* avepst_d = nullptr;
@@ -149,12 +148,14 @@ int update(pfdd_ctx* c) {
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
 #define PFDD_UP(MM, TT)                                                                                  \
   hipLaunchKernelGGL((k_update<T, MM, TT>), grid, dim3(UPD_THREADS), 0, c->stream, c->xi, W, c->mask, c->taupp, \
-                     c->sa, c->partials, c->counter, c->hist, c->step_ctr)
+                     c->sa, c->partials)
   if (m && tp) PFDD_UP(true, true);
   else if (m) PFDD_UP(true, false);
   else if (tp) PFDD_UP(false, true);
   else PFDD_UP(false, false);
 #undef PFDD_UP
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->upd_nblk, c->hist,
+                     c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -242,8 +243,9 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
   // 1. <xi_a> (avestrain G.c:670-686)
   {
     dim3 grid((unsigned)c->upd_nblk, (unsigned)S);
-    hipLaunchKernelGGL(k_xi_sums, grid, dim3(UPD_THREADS), 0, c->stream, c->xi, P, S, c->upd_nblk, c->partials,
-                       c->counter, c->small);
+    hipLaunchKernelGGL(k_xi_sums, grid, dim3(UPD_THREADS), 0, c->stream, c->xi, P, c->upd_nblk, c->partials);
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, S, c->upd_nblk, c->small,
+                       (int*)nullptr);
     HIPCK(hipPeekAtLastError());
   }
   double sums[MAXS + 1];
@@ -271,9 +273,11 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
   double* sf = fields ? c->sfield : nullptr;
 #define PFDD_SR(SM)                                                                                       \
   hipLaunchKernelGGL((k_strain_stress<T, SM>), dim3(nb), dim3(256), 0, c->stream, E, c->xi, c->ec, c->avepst_d, P, \
-                     c->partials, (int)nb, c->counter, c->small, ef, sf)
+                     c->partials, (int)nb, ef, sf)
   if (S <= 4) PFDD_SR(4); else if (S <= 8) PFDD_SR(8); else PFDD_SR(16);
 #undef PFDD_SR
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, 18, (int)nb, c->small,
+                     (int*)nullptr);
   HIPCK(hipPeekAtLastError());
   double out[19];
   HIPCK(hipMemcpyAsync(out, c->small, sizeof(double) * 19, hipMemcpyDeviceToHost, c->stream));
@@ -311,7 +315,7 @@ void pfdd_destroy(pfdd_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
-  void* bufs[] = {c->xi, c->W, c->Bsym, c->mask, c->taupp, c->partials, c->hist, c->counter, c->step_ctr,
+  void* bufs[] = {c->xi, c->W, c->Bsym, c->mask, c->taupp, c->partials, c->hist, c->step_ctr,
                   c->small, c->avepst_d, c->ec, c->E6, c->efield, c->sfield};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -365,12 +369,10 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     if (!(c->W = dev_alloc<char>(csz * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->hist = (double*)dev_alloc<char>(sizeof(double) * HIST_CAP * (S + 1), c->bytes))) FAIL(PFDD_E_ALLOC);
-    if (!(c->counter = (unsigned*)dev_alloc<char>(sizeof(unsigned) * 4, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->step_ctr = (int*)dev_alloc<char>(sizeof(int) * 4, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->small = (double*)dev_alloc<char>(sizeof(double) * 32, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->avepst_d = (double*)dev_alloc<char>(sizeof(double) * 9, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->ec = (ElasticConsts*)dev_alloc<char>(sizeof(ElasticConsts), c->bytes))) FAIL(PFDD_E_ALLOC);
-    if (hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 4, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if (hipMemsetAsync(c->step_ctr, 0, sizeof(int) * 4, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if (hipMemsetAsync(c->xi, 0, sizeof(double2) * S * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if (hipMemsetAsync(c->W, 0, csz * S * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);

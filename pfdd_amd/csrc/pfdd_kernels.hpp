@@ -167,27 +167,20 @@ struct StepArgs {
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 
-// Last-block-done finish of a per-field sum: partials[f][nblk] -> out[f]
-// (and out[ns] = sum over fields).  The counter resets itself.
-__device__ __forceinline__ void finish_field_sums(const double* partials, int ns, int nblk,
-                                                  unsigned* counter, double* out, int* step_ctr,
-                                                  double* red) {
-  __shared__ bool last;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    const unsigned total = (unsigned)(gridDim.x * gridDim.y);
-    last = (atomicAdd(counter, 1u) == total - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double* vp = partials;
+// Second pass of the per-field sums: one block of FIN_THREADS threads adds
+// partials[f][0..nblk) for every field in a fixed order and writes out[f] and
+// out[ns] = sum over fields; with step_ctr the row is out + (*step_ctr)(ns+1)
+// and the counter advances (one row per GL step).
+constexpr int FIN_THREADS = 1024;
+__global__ void __launch_bounds__(FIN_THREADS)
+k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out, int* step_ctr) {
+  __shared__ double red[FIN_THREADS / 64];
   double* dst = out;
   if (step_ctr) dst = out + (long)(*step_ctr) * (ns + 1);
   double tot = 0.0;
   for (int f = 0; f < ns; f++) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) v += vp[(long)f * nblk + b];
+    for (int b = threadIdx.x; b < nblk; b += FIN_THREADS) v += partials[(long)f * nblk + b];
     const double s = block_sum(v, red);
     if (threadIdx.x == 0) {
       dst[f] = s;
@@ -197,7 +190,6 @@ __device__ __forceinline__ void finish_field_sums(const double* partials, int ns
   if (threadIdx.x == 0) {
     dst[ns] = tot;
     if (step_ctr) *step_ctr = *step_ctr + 1;
-    *counter = 0u;
   }
 }
 
@@ -206,8 +198,7 @@ __device__ __forceinline__ void finish_field_sums(const double* partials, int ns
 template <typename T, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__(UPD_THREADS)
 k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
-         const double* __restrict__ taupp, StepArgs a, double* partials, unsigned* counter,
-         double* hist, int* step_ctr) {
+         const double* __restrict__ taupp, StepArgs a, double* __restrict__ partials) {
   __shared__ double red[UPD_THREADS / 64];
   const int f = blockIdx.y;
   const long P = a.P;
@@ -234,13 +225,11 @@ k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restr
   }
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0) partials[(long)f * a.nblk + blockIdx.x] = s;
-  finish_field_sums(partials, a.ns, a.nblk, counter, hist, step_ctr, red);
 }
 
 // Sum of Re xi per field (avestrain's xi means, G.c:670-686).  grid = (nblk, S).
 __global__ void __launch_bounds__(UPD_THREADS)
-k_xi_sums(const double2* __restrict__ xi, long P, int ns, int nblk, double* partials,
-          unsigned* counter, double* out) {
+k_xi_sums(const double2* __restrict__ xi, long P, int nblk, double* __restrict__ partials) {
   __shared__ double red[UPD_THREADS / 64];
   const int f = blockIdx.y;
   const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
@@ -251,7 +240,6 @@ k_xi_sums(const double2* __restrict__ xi, long P, int ns, int nblk, double* part
   }
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0) partials[(long)f * nblk + blockIdx.x] = s;
-  finish_field_sums(partials, ns, nblk, counter, out, nullptr, red);
 }
 
 template <typename T>
@@ -412,8 +400,8 @@ k_strain_spectrum(const cx<T>* __restrict__ W, cx<T>* __restrict__ E, const Elas
 template <typename T, int SM>
 __global__ void __launch_bounds__(256)
 k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, const ElasticConsts* __restrict__ ec,
-                const double* __restrict__ avepst, long P, double* partials, int nblk, unsigned* counter,
-                double* out, double* eps_field, double* sig_field) {
+                const double* __restrict__ avepst, long P, double* __restrict__ partials, int nblk,
+                double* eps_field, double* sig_field) {
   __shared__ double red[4];
   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int S = ec->ns;
@@ -451,7 +439,6 @@ k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, con
     const double s = block_sum(c < 9 ? e[c] : sg[c - 9], red);
     if (threadIdx.x == 0) partials[(long)c * nblk + blockIdx.x] = s;
   }
-  finish_field_sums(partials, 18, nblk, counter, out, nullptr, red);
 }
 
 }  // namespace pfdd
