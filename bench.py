@@ -69,7 +69,14 @@ def timed_steps(ctx, steps, warmup):
 
 def time_stages(ctx, reps=20):
     out = {}
-    for name, st, per_step in STAGES + [("k_xi_to_w(copy)", pa.STAGE_COPY, 0)]:
+    plan = ctx.step_plan()
+    names = {st: n for n, st, _ in STAGES}
+    if plan[pa.STAGE_FFT_AXIS3] == 0:
+        names[pa.STAGE_UPDATE] = "k_row_step(N3 inv + update + N3 fwd)"
+    if plan[pa.STAGE_FFT_AXIS1] == 0:
+        names[pa.STAGE_CONTRACT] = "k_col_contract(N1 fwd + B(k) + N1 inv)"
+    todo = [(names[st], st, plan[st]) for _, st, _ in STAGES] + [("k_xi_to_w(copy)", pa.STAGE_COPY, 0)]
+    for name, st, per_step in todo:
         ctx.enqueue_stage(st, 2)
         e0, e1 = hip.Event(), hip.Event()
         e0.record(ctx.stream)

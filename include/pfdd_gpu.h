@@ -127,15 +127,20 @@ long long pfdd_device_bytes(const pfdd_ctx *c);
  * rocprofv3 can time it in isolation.  It overwrites the xi / work-array
  * state (upload xi again afterwards).  Asynchronous. */
 enum {
-  PFDD_STAGE_UPDATE = 0,    /* plasticevolv + copy + gamma sums */
+  PFDD_STAGE_UPDATE = 0,    /* plasticevolv + copy + gamma sums (fused: + N3 passes) */
   PFDD_STAGE_FFT_AXIS3 = 1, /* one pass along N3 (contiguous) over all fields */
   PFDD_STAGE_FFT_AXIS2 = 2, /* one pass along N2 */
   PFDD_STAGE_FFT_AXIS1 = 3, /* one pass along N1 */
-  PFDD_STAGE_CONTRACT = 4,  /* B(k) contraction */
+  PFDD_STAGE_CONTRACT = 4,  /* B(k) contraction (fused: + N1 passes) */
   PFDD_STAGE_STEP = 5,      /* one whole GL step */
   PFDD_STAGE_COPY = 6       /* plain copy xi -> work array (bandwidth reference) */
 };
 int pfdd_enqueue_stage(pfdd_ctx *c, int stage, int reps);
+
+/* Launches of each stage per GL step for this context (launches[7]):
+ * PFDD_STAGE_UPDATE and PFDD_STAGE_CONTRACT name the fused row and column
+ * kernels when the grid allows them, and the passes they absorb count 0. */
+int pfdd_step_plan(const pfdd_ctx *c, int *launches);
 
 const char *pfdd_strerror(int rc);
 
@@ -169,6 +174,33 @@ double pfdd_uniform01(unsigned long long *state);
  * Returns the number of points set. */
 long pfdd_add_loop(double *xi, int n1, int n2, int n3, int sys, const double normal[3],
                    const double center[3], double radius);
+
+/* Seeded loops of the benchmark workloads: for each of nloops loops, draw
+ * (in this order from the SplitMix64 stream) the radius r ~ U[rmin, rmax),
+ * the system a = floor(U ns), and the centre c_i ~ U[0, N_i); then
+ * pfdd_add_loop(xi, ..., a, normals[a], c, r).  loops_out (NULL or
+ * [nloops][6]) receives a, c, r and the number of points set. */
+int pfdd_seed_loops(double *xi, int n1, int n2, int n3, int ns, const double (*normals)[3], int nloops,
+                    double rmin, double rmax, unsigned long long *rng_state, double *loops_out);
+
+/* Symmetric applied stress with the six independent components (11, 12, 13,
+ * 22, 23, 33) drawn in that order from U[-amp, amp). */
+void pfdd_seed_sigma(double amp, unsigned long long *rng_state, double sigma[3][3]);
+
+/* ---- the reference's FFT seam on host arrays ----------------------------
+ * fft_forward / fft_backward have the exact signature and semantics of
+ * G.c:209-273 (in place, unnormalised, batch contiguous N1*N2*N3 complex
+ * float blocks, forward sign -1) and run on the GPU: each call copies the
+ * batch to the device, transforms it in FP32 and copies it back.
+ * pfdd_fft_host_init fixes the grid (the reference's N1..N3); it must be
+ * called first and again to change the grid.  Returns 0 or a PFDD_E_* code;
+ * the seam functions themselves return nothing, as in the reference, and
+ * pfdd_fft_host_status() reports the last error. */
+int pfdd_fft_host_init(int n1, int n2, int n3);
+void fft_forward(float *data, int batch);
+void fft_backward(float *data, int batch);
+int pfdd_fft_host_status(void);
+void pfdd_fft_host_release(void);
 
 #ifdef __cplusplus
 }

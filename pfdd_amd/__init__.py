@@ -24,6 +24,8 @@ EXPORTED = [
     "pfdd_begin_outer", "pfdd_gl_steps", "pfdd_strain_stress", "pfdd_fft", "pfdd_sync", "pfdd_stream",
     "pfdd_device_bytes", "pfdd_strerror", "pfdd_setfcc", "pfdd_setorient", "pfdd_seteps",
     "pfdd_resolve_tau", "pfdd_splitmix64", "pfdd_uniform01", "pfdd_add_loop", "pfdd_enqueue_stage",
+    "pfdd_step_plan", "pfdd_seed_loops", "pfdd_seed_sigma", "pfdd_fft_host_init", "fft_forward", "fft_backward",
+    "pfdd_fft_host_status", "pfdd_fft_host_release",
 ]
 STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY = range(7)
 
@@ -91,6 +93,18 @@ def load():
     L.pfdd_add_loop.argtypes = [vp, ci, ci, ci, ci, vp, vp, cd]
     L.pfdd_add_loop.restype = cl
     L.pfdd_enqueue_stage.argtypes = [vp, ci, ci]
+    L.pfdd_step_plan.argtypes = [vp, vp]
+    L.pfdd_seed_loops.argtypes = [vp, ci, ci, ci, ci, vp, ci, cd, cd, ctypes.POINTER(ctypes.c_ulonglong), vp]
+    L.pfdd_seed_sigma.argtypes = [cd, ctypes.POINTER(ctypes.c_ulonglong), vp]
+    L.pfdd_seed_sigma.restype = None
+    L.pfdd_fft_host_init.argtypes = [ci, ci, ci]
+    L.fft_forward.argtypes = [vp, ci]
+    L.fft_forward.restype = None
+    L.fft_backward.argtypes = [vp, ci]
+    L.fft_backward.restype = None
+    L.pfdd_fft_host_status.argtypes = []
+    L.pfdd_fft_host_release.argtypes = []
+    L.pfdd_fft_host_release.restype = None
     _lib = L
     return L
 
@@ -203,6 +217,11 @@ class Context:
 
     def enqueue_stage(self, stage, reps):
         _check(load().pfdd_enqueue_stage(self.h, stage, reps), "pfdd_enqueue_stage")
+
+    def step_plan(self):
+        a = (ctypes.c_int * 7)()
+        _check(load().pfdd_step_plan(self.h, a), "pfdd_step_plan")
+        return list(a)
 
     @property
     def stream(self):

@@ -149,21 +149,21 @@ def _fcc_systems(systems, theta):
 
 
 def _random_loops(rng, xi, normals, nloops, rmin, rmax):
+    """pfdd_seed_loops: per loop r, system, centre (x, y, z) from the stream."""
     ns, n1, n2, n3 = xi.shape
-    loops = []
-    for _ in range(nloops):
-        r = rmin + (rmax - rmin) * rng.uniform()
-        a = min(int(rng.uniform() * ns), ns - 1)
-        c = (rng.uniform() * n1, rng.uniform() * n2, rng.uniform() * n3)
-        cnt = add_loop(xi, a, normals[a], c, r)
-        loops.append((a, c, r, cnt))
-    return loops
+    assert xi.dtype == np.complex128 and xi.flags.c_contiguous
+    nrm = np.ascontiguousarray(normals, dtype=np.float64)
+    out = np.zeros((max(nloops, 1), 6))
+    rc = load().pfdd_seed_loops(xi.ctypes.data, n1, n2, n3, ns, nrm.ctypes.data, nloops, float(rmin), float(rmax),
+                                ctypes.byref(rng.s), out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"pfdd_seed_loops failed: {rc}")
+    return [(int(o[0]), (o[1], o[2], o[3]), o[4], int(o[5])) for o in out[:nloops]]
 
 
 def _random_sigma(rng, amp):
     s = np.zeros((3, 3))
-    for (i, j) in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)):
-        s[i, j] = s[j, i] = -amp + 2 * amp * rng.uniform()
+    load().pfdd_seed_sigma(float(amp), ctypes.byref(rng.s), s.ctypes.data)
     return s
 
 

@@ -65,6 +65,10 @@ struct pfdd_ctx {
   Twiddles tw[3];
   StepArgs sa{};
   int upd_nblk = 0;
+  int row_tile = 0;          // rows per block of the fused row kernel (0: not fused)
+  int row_nblk = 0;          // its blocks per field
+  int c3_tile = 0;           // columns per block of the fused column kernel (0: not fused)
+  size_t c3_lds = 0;         // its dynamic LDS bytes
   int has_xi = 0;
   int prologue_done = 0;
   hipGraphExec_t graph = nullptr;
@@ -136,7 +140,7 @@ int contract(pfdd_ctx* c, int quirk) {
     hipLaunchKernelGGL((k_contract<T, SM, true>), dim3(nb), dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
   else                                                                                                  \
     hipLaunchKernelGGL((k_contract<T, SM, false>), dim3(nb), dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
-  if (c->S <= 4) { PFDD_CT(4) } else if (c->S <= 8) { PFDD_CT(8) } else { PFDD_CT(16) }
+  if (c->S <= 4) { PFDD_CT(4) } else if (c->S <= 8) { PFDD_CT(8) } else if (c->S <= 12) { PFDD_CT(12) } else { PFDD_CT(16) }
 #undef PFDD_CT
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
@@ -159,20 +163,142 @@ int update(pfdd_ctx* c) {
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
-template <typename T>
-int elastic_solve(pfdd_ctx* c, int quirk) {
-  int rc;
+constexpr int kFusedLens[] = {32, 48, 64, 96, 128, 256, 512};
+constexpr size_t C3_LDS_BUDGET = 104 * 1024;
+// The fused column kernel holds S*tile pencils in LDS; at NS=12 that allows
+// only 4 (FP64) or 8 (FP32) columns and one block per CU, and it measures
+// slower than the three unfused passes (0.71 vs 0.61 ms FP64 at 128^3x12).
+// It is used only when it gets at least this many columns per block.
+constexpr int C3_MIN_TILE = 16;
+
+bool fused_len(int n) {
+  for (int v : kFusedLens)
+    if (v == n) return true;
+  return false;
+}
+
+int row_tile_max(int n, bool fp64) {
+  switch (n) {
+#define PFDD_L(NN) case NN: return fp64 ? tile_rows<double, NN>() : tile_rows<float, NN>();
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+#undef PFDD_L
+    default: return 0;
+  }
+}
+
+template <typename T, int N>
+void k1_launch_n(pfdd_ctx* c) {
   cx<T>* W = (cx<T>*)c->W;
-  if ((rc = fft3d<T>(c, W, c->S, PFDD_FORWARD))) return rc;
-  if ((rc = contract<T>(c, quirk))) return rc;
-  return fft3d<T>(c, W, c->S, PFDD_BACKWARD);
+  const long nb = (long)c->S * c->row_nblk;
+  constexpr int NT = tile_rows<T, N>() * make_plan(N).tpp;
+  const cx<T>* tw = tw_of<T>(c->tw[2]);
+#define PFDD_K1(MM, TT)                                                                                   \
+  hipLaunchKernelGGL((k_row_step<T, N, MM, TT>), dim3((unsigned)nb), dim3(NT), 0, c->stream, W, c->xi, c->mask, \
+                     c->taupp, c->sa, c->row_tile, tw, c->partials)
+  const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+  if (m && tp) PFDD_K1(true, true);
+  else if (m) PFDD_K1(true, false);
+  else if (tp) PFDD_K1(false, true);
+  else PFDD_K1(false, false);
+#undef PFDD_K1
 }
 
 template <typename T>
+int k1_launch(pfdd_ctx* c) {
+  switch (c->p.n3) {
+#define PFDD_L(NN) case NN: k1_launch_n<T, NN>(c); break;
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+#undef PFDD_L
+    default: return PFDD_E_RADIX;
+  }
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->row_nblk, c->hist,
+                     c->step_ctr);
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+template <typename T, int N, int SM>
+const void* k3_symbol() {
+  return reinterpret_cast<const void*>(&k_col_contract<T, N, SM>);
+}
+
+template <typename T, int N>
+int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
+  const long nb = ((long)c->p.n2 * c->p.n3) / c->c3_tile;
+  cx<T>* W = (cx<T>*)c->W;
+  const cx<T>* tw = tw_of<T>(c->tw[0]);
+#define PFDD_K3(SM)                                                                                             \
+  if (attr_only) {                                                                                              \
+    HIPCK(hipFuncSetAttribute(k3_symbol<T, N, SM>(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->c3_lds)); \
+  } else {                                                                                                      \
+    hipLaunchKernelGGL((k_col_contract<T, N, SM>), dim3((unsigned)nb), dim3(C3_THREADS), c->c3_lds, c->stream, W,      \
+                       c->ec, c->P, c->c3_tile, quirk, tw);                                                     \
+  }
+  if (c->S <= 4) { PFDD_K3(4) } else if (c->S <= 8) { PFDD_K3(8) } else if (c->S <= 12) { PFDD_K3(12) } else { PFDD_K3(16) }
+#undef PFDD_K3
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+template <typename T>
+int k3_launch(pfdd_ctx* c, int quirk, bool attr_only = false) {
+  switch (c->p.n1) {
+#define PFDD_L(NN) case NN: return k3_launch_n<T, NN>(c, quirk, attr_only);
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+#undef PFDD_L
+    default: return PFDD_E_RADIX;
+  }
+}
+
+// Passes between the forward N3 pass and the inverse N3 pass: forward N2,
+// then (forward N1, contraction, inverse N1) -- fused when it fits -- then
+// inverse N2.
+template <typename T>
+int middle(pfdd_ctx* c, int quirk) {
+  int rc;
+  cx<T>* W = (cx<T>*)c->W;
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
+  if ((rc = launch_axis<T>(W, (long)S * n1, n2, n3, PFDD_FORWARD, tw_of<T>(c->tw[1]), c->stream))) return rc;
+  if (c->c3_tile > 0) {
+    if ((rc = k3_launch<T>(c, quirk))) return rc;
+  } else {
+    if ((rc = launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_FORWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
+    if ((rc = contract<T>(c, quirk))) return rc;
+    if ((rc = launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
+  }
+  return launch_axis<T>(W, (long)S * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream);
+}
+
+template <typename T>
+int axis3(pfdd_ctx* c, int sign) {
+  return launch_axis<T>((cx<T>*)c->W, (long)c->S * c->p.n1 * c->p.n2, c->p.n3, 1, sign, tw_of<T>(c->tw[2]),
+                        c->stream);
+}
+
+// Between GL steps W holds data2 with the inverse transform done along N1
+// and N2; the N3 pass is finished by the next step's row kernel (or, when
+// the row kernel is not fused, W holds data2 fully in real space).
+template <typename T>
 int one_step(pfdd_ctx* c) {
   int rc;
-  if ((rc = update<T>(c))) return rc;
-  return elastic_solve<T>(c, 0);
+  if (c->row_tile > 0) {
+    if ((rc = k1_launch<T>(c))) return rc;
+  } else {
+    if ((rc = update<T>(c))) return rc;
+    if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
+  }
+  if ((rc = middle<T>(c, 0))) return rc;
+  if (c->row_tile == 0) return axis3<T>(c, PFDD_BACKWARD);
+  return PFDD_OK;
+}
+
+template <typename T>
+int prologue(pfdd_ctx* c, int quirk) {
+  int rc;
+  const long n = (long)c->S * c->P;
+  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, (cx<T>*)c->W, n);
+  if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
+  if ((rc = middle<T>(c, quirk))) return rc;
+  if (c->row_tile == 0) return axis3<T>(c, PFDD_BACKWARD);
+  return PFDD_OK;
 }
 
 int step(pfdd_ctx* c) { return c->fp64 ? one_step<double>(c) : one_step<float>(c); }
@@ -364,7 +490,30 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     const int S = c->S;
     c->upd_nblk = (int)((P + UPD_THREADS * UPD_EPT - 1) / (UPD_THREADS * UPD_EPT));
     const long strain_nblk = (P + 255) / 256;
-    c->partials_len = (size_t)std::max<long>((long)S * c->upd_nblk, 18 * strain_nblk);
+    // fused row kernel: rows per block = the largest power of two <= its
+    // compile-time tile that divides the N1*N2 rows of a field
+    if (fused_len(p->n3)) {
+      const long rows = (long)p->n1 * p->n2;
+      const int tmax = row_tile_max(p->n3, c->fp64);
+      int t = 1;
+      while (t * 2 <= tmax && rows % (t * 2) == 0) t *= 2;
+      c->row_tile = t;
+      c->row_nblk = (int)(rows / t);
+    }
+    // fused column kernel: S*tile pencils of N1 points in LDS
+    if (fused_len(p->n1) && !c->stored) {
+      const size_t per_col = (size_t)S * (p->n1 + 1) * csz;
+      const int maxpen = (c->fp64 ? 1 : 2) * (C3_THREADS / make_plan(p->n1).tpp);
+      const long inner = (long)p->n2 * p->n3;
+      int t = 0;
+      for (int cand = 1; cand * S <= maxpen && cand * per_col <= C3_LDS_BUDGET && inner % cand == 0; cand *= 2) t = cand;
+      if (t >= C3_MIN_TILE) {
+        c->c3_tile = t;
+        c->c3_lds = t * per_col;
+      }
+    }
+    c->partials_len = (size_t)std::max<long>(std::max<long>((long)S * c->upd_nblk, (long)S * c->row_nblk),
+                                             18 * strain_nblk);
     if (!(c->xi = (double2*)dev_alloc<char>(sizeof(double2) * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->W = dev_alloc<char>(csz * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -436,6 +585,10 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
       }
 #undef PFDD_BB
       if (hipPeekAtLastError() != hipSuccess) FAIL(PFDD_E_HIP);
+    }
+    if (c->c3_tile > 0) {
+      rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true);
+      if (rc) goto fail;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
   }
@@ -525,16 +678,7 @@ int pfdd_download_xi(pfdd_ctx* c, double* xi, int nsv) {
 int pfdd_begin_outer(pfdd_ctx* c, int quirk) {
   if (!c) return PFDD_E_SHAPE;
   if (!c->has_xi) return PFDD_E_STATE;
-  const long n = (long)c->S * c->P;
-  const unsigned nb = (unsigned)((n + 255) / 256);
-  int rc;
-  if (c->fp64) {
-    hipLaunchKernelGGL((k_xi_to_w<double>), dim3(nb), dim3(256), 0, c->stream, c->xi, (double2*)c->W, n);
-    rc = elastic_solve<double>(c, quirk ? 1 : 0);
-  } else {
-    hipLaunchKernelGGL((k_xi_to_w<float>), dim3(nb), dim3(256), 0, c->stream, c->xi, (float2*)c->W, n);
-    rc = elastic_solve<float>(c, quirk ? 1 : 0);
-  }
+  const int rc = c->fp64 ? prologue<double>(c, quirk ? 1 : 0) : prologue<float>(c, quirk ? 1 : 0);
   if (rc) return rc;
   c->prologue_done = 1;
   return PFDD_OK;
@@ -631,7 +775,7 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
     switch (stage) {
       case PFDD_STAGE_UPDATE:
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
-        rc = update<T>(c);
+        rc = c->row_tile > 0 ? k1_launch<T>(c) : update<T>(c);
         break;
       case PFDD_STAGE_FFT_AXIS3:
         rc = launch_axis<T>(W, (long)S * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream);
@@ -643,7 +787,7 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
         rc = launch_axis<T>(W, (long)S, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream);
         break;
       case PFDD_STAGE_CONTRACT:
-        rc = contract<T>(c, 0);
+        rc = c->c3_tile > 0 ? k3_launch<T>(c, 0) : contract<T>(c, 0);
         break;
       case PFDD_STAGE_STEP:
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
@@ -662,8 +806,91 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
 
 }  // namespace
 
+extern "C" int pfdd_step_plan(const pfdd_ctx* c, int* launches) {
+  if (!c || !launches) return PFDD_E_SHAPE;
+  const bool r = c->row_tile > 0, k3 = c->c3_tile > 0;
+  launches[PFDD_STAGE_UPDATE] = 1;
+  launches[PFDD_STAGE_FFT_AXIS3] = r ? 0 : 2;
+  launches[PFDD_STAGE_FFT_AXIS2] = 2;
+  launches[PFDD_STAGE_FFT_AXIS1] = k3 ? 0 : 2;
+  launches[PFDD_STAGE_CONTRACT] = 1;
+  launches[PFDD_STAGE_STEP] = 0;
+  launches[PFDD_STAGE_COPY] = 0;
+  return PFDD_OK;
+}
+
 extern "C" int pfdd_enqueue_stage(pfdd_ctx* c, int stage, int reps) {
   if (!c || reps < 0 || stage < PFDD_STAGE_UPDATE || stage > PFDD_STAGE_COPY) return PFDD_E_SHAPE;
   c->prologue_done = 0;
   return c->fp64 ? enqueue_stage<double>(c, stage, reps) : enqueue_stage<float>(c, stage, reps);
 }
+
+// ---------------------------------------------------------------- FFT seam on host arrays
+// Process-wide, like the static plans of G.c:124,170; not thread-safe.
+namespace {
+struct HostSeam {
+  pfdd_ctx* ctx = nullptr;
+  float2* buf = nullptr;
+  size_t cap = 0;  // complex elements
+  int status = PFDD_E_STATE;
+};
+HostSeam g_seam;
+
+int seam_run(float* data, int batch, int dir) {
+  if (!g_seam.ctx) return PFDD_E_STATE;
+  if (!data || batch < 1) return PFDD_E_SHAPE;
+  pfdd_ctx* c = g_seam.ctx;
+  const size_t n = (size_t)batch * c->P;
+  if (n > g_seam.cap) {
+    if (g_seam.buf) (void)hipFree(g_seam.buf);
+    g_seam.buf = nullptr;
+    g_seam.cap = 0;
+    if (hipMalloc(&g_seam.buf, n * sizeof(float2)) != hipSuccess) return PFDD_E_ALLOC;
+    g_seam.cap = n;
+  }
+  HIPCK(hipMemcpyAsync(g_seam.buf, data, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  const int rc = fft3d<float>(c, g_seam.buf, batch, dir);
+  if (rc) return rc;
+  HIPCK(hipMemcpyAsync(data, g_seam.buf, n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return PFDD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+void pfdd_fft_host_release(void) {
+  if (g_seam.buf) (void)hipFree(g_seam.buf);
+  g_seam.buf = nullptr;
+  g_seam.cap = 0;
+  pfdd_destroy(g_seam.ctx);
+  g_seam.ctx = nullptr;
+  g_seam.status = PFDD_E_STATE;
+}
+
+int pfdd_fft_host_init(int n1, int n2, int n3) {
+  pfdd_fft_host_release();
+  pfdd_params p{};
+  p.n1 = n1;
+  p.n2 = n2;
+  p.n3 = n3;
+  p.ns = 1;
+  p.precision = PFDD_MIXED;
+  p.bmode = PFDD_B_RECOMPUTE;
+  p.C11 = 3.0;
+  p.C12 = 1.0;
+  p.C44 = 1.0;
+  p.d1 = p.d2 = p.d3 = 1.0;
+  p.cd2[0] = 0.1;
+  p.dslip[0] = 1.0;
+  p.uq2 = 1.0;
+  p.pi_const = 3.141592654;
+  g_seam.status = pfdd_create(&p, &g_seam.ctx);
+  return g_seam.status;
+}
+
+void fft_forward(float* data, int batch) { g_seam.status = seam_run(data, batch, PFDD_FORWARD); }
+void fft_backward(float* data, int batch) { g_seam.status = seam_run(data, batch, PFDD_BACKWARD); }
+int pfdd_fft_host_status(void) { return g_seam.status; }
+
+}  // extern "C"

@@ -120,59 +120,83 @@ template <typename T> struct Bfly<T, 8> {
 };
 
 // ---------------------------------------------------------------- passes
-// One Stockham pass on a pencil of N points at sm (LDS).  q in [0, TPP).
-// tw[m] = exp(+2 pi i m / N); the sign flips its imaginary part.
-template <typename T, int N, int R, int NSB, int TPP>
-__device__ __forceinline__ void stockham_pass(cx<T>* sm, int q, T s, const cx<T>* __restrict__ tw) {
+// One Stockham pass on G pencils per thread.  Pencil g of this thread starts
+// at sm + (p0 + g * pstride) * LD and is processed when p0 + g * pstride < npen;
+// q in [0, TPP) is the thread's lane within the pencil.  tw[m] = exp(+2 pi i m / N);
+// the sign s flips its imaginary part.  Every thread of the block must call it.
+template <typename T, int N, int R, int NSB, int TPP, int G, int LD>
+__device__ __forceinline__ void stockham_pass(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
+                                              const cx<T>* __restrict__ tw) {
   constexpr int M = N / R;
   constexpr int J = (M + TPP - 1) / TPP;
   constexpr int STEP = N / (NSB * R);
-  cx<T> v[J][R];
+  cx<T> v[G][J][R];
 #pragma unroll
-  for (int jj = 0; jj < J; ++jj) {
-    const int j = q + jj * TPP;
-    if ((M % TPP == 0) || j < M) {
+  for (int g = 0; g < G; ++g) {
+    const int p = p0 + g * pstride;
+    if (p >= npen) continue;
+    cx<T>* pen = sm + p * LD;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[jj][r] = sm[j + r * M];
-      if (NSB > 1) {
-        const int k = j % NSB;
+    for (int jj = 0; jj < J; ++jj) {
+      const int j = q + jj * TPP;
+      if ((M % TPP == 0) || j < M) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          cx<T> w = tw[k * r * STEP];
-          w.y *= s;
-          v[jj][r] = cmul<T>(v[jj][r], w);
+        for (int r = 0; r < R; ++r) v[g][jj][r] = pen[j + r * M];
+        if (NSB > 1) {
+          const int k = j % NSB;
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            cx<T> w = tw[k * r * STEP];
+            w.y *= s;
+            v[g][jj][r] = cmul<T>(v[g][jj][r], w);
+          }
         }
+        Bfly<T, R>::run(v[g][jj], s);
       }
-      Bfly<T, R>::run(v[jj], s);
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int jj = 0; jj < J; ++jj) {
-    const int j = q + jj * TPP;
-    if ((M % TPP == 0) || j < M) {
-      const int d = (j / NSB) * NSB * R + (j % NSB);
+  for (int g = 0; g < G; ++g) {
+    const int p = p0 + g * pstride;
+    if (p >= npen) continue;
+    cx<T>* pen = sm + p * LD;
 #pragma unroll
-      for (int r = 0; r < R; ++r) sm[d + r * NSB] = v[jj][r];
+    for (int jj = 0; jj < J; ++jj) {
+      const int j = q + jj * TPP;
+      if ((M % TPP == 0) || j < M) {
+        const int d = (j / NSB) * NSB * R + (j % NSB);
+#pragma unroll
+        for (int r = 0; r < R; ++r) pen[d + r * NSB] = v[g][jj][r];
+      }
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int K>
-__device__ __forceinline__ void fft_lds_from(cx<T>* sm, int q, T s, const cx<T>* __restrict__ tw) {
+template <typename T, int N, int G, int LD, int K>
+__device__ __forceinline__ void fft_lds_from(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
+                                             const cx<T>* __restrict__ tw) {
   constexpr Plan pl = make_plan(N);
   if constexpr (K < pl.np) {
-    stockham_pass<T, N, pl.r[K], ns_before(N, K), pl.tpp>(sm, q, s, tw);
-    fft_lds_from<T, N, K + 1>(sm, q, s, tw);
+    stockham_pass<T, N, pl.r[K], ns_before(N, K), pl.tpp, G, LD>(sm, p0, pstride, npen, q, s, tw);
+    fft_lds_from<T, N, G, LD, K + 1>(sm, p0, pstride, npen, q, s, tw);
   }
 }
 
-// Full N-point transform of the pencil at sm; every thread of the block must
-// call it (it synchronises).  The caller synchronises after filling sm.
-template <typename T, int N>
-__device__ __forceinline__ void fft_lds(cx<T>* sm, int q, T s, const cx<T>* __restrict__ tw) {
-  fft_lds_from<T, N, 0>(sm, q, s, tw);
+// N-point transforms of up to G pencils per thread (pencil p at sm + p*LD).
+// The caller synchronises after filling sm; the function ends synchronised.
+template <typename T, int N, int G, int LD>
+__device__ __forceinline__ void fft_lds_multi(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
+                                              const cx<T>* __restrict__ tw) {
+  fft_lds_from<T, N, G, LD, 0>(sm, p0, pstride, npen, q, s, tw);
+}
+
+// One pencil per thread group: pencil p = threadIdx.x / TPP at sm + p*LD.
+template <typename T, int N, int LD>
+__device__ __forceinline__ void fft_lds(cx<T>* sm, int npen, T s, const cx<T>* __restrict__ tw) {
+  constexpr int TPP = make_plan(N).tpp;
+  fft_lds_from<T, N, 1, LD, 0>(sm, (int)threadIdx.x / TPP, 1 << 30, npen, (int)threadIdx.x % TPP, s, tw);
 }
 
 // LDS row pitch (in complex elements) for a pencil of N points.

@@ -97,17 +97,23 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 // ---------------------------------------------------------------- FFT kernels
+constexpr int pow2_floor(int v) {
+  int t = 1;
+  while (t * 2 <= v) t *= 2;
+  return t;
+}
+// Rows per block of the contiguous-axis kernels: a power of two, so that the
+// block size TILE*TPP is a multiple of 64 whenever it reaches 64.
 template <typename T, int N> constexpr int tile_rows() {
   constexpr int tpp = make_plan(N).tpp;
-  int t = 256 / tpp;
-  if (t < 1) t = 1;
+  int t = pow2_floor(256 / tpp > 0 ? 256 / tpp : 1);
   const int lim = 32768 / (pitch<N>() * (int)sizeof(cx<T>));
   while (t > lim && t > 1) t /= 2;
   return t;
 }
 template <typename T, int N> constexpr int tile_cols() {
   constexpr int tpp = make_plan(N).tpp;
-  int t = 256 / tpp;
+  int t = pow2_floor(256 / tpp > 0 ? 256 / tpp : 1);
   if (t < 16) t = 16;
   const int lim = 65536 / (pitch<N>() * (int)sizeof(cx<T>));
   while (t > lim && t > 1) t /= 2;
@@ -126,8 +132,7 @@ k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict
   cx<T>* base = data + row0 * N;
   for (int e = threadIdx.x; e < nr * N; e += NT) sm[(e / N) * LD + (e % N)] = base[e];
   __syncthreads();
-  const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
-  fft_lds<T, N>(sm + p * LD, q, sign, tw);
+  fft_lds<T, N, LD>(sm, nr, sign, tw);
   for (int e = threadIdx.x; e < nr * N; e += NT) base[e] = sm[(e / N) * LD + (e % N)];
 }
 
@@ -147,8 +152,7 @@ k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* 
     sm[p * LD + n] = base[(long)n * inner + p];
   }
   __syncthreads();
-  const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
-  fft_lds<T, N>(sm + p * LD, q, sign, tw);  // pencils p >= tile transform garbage, never stored
+  fft_lds<T, N, LD>(sm, tile, sign, tw);
   for (int e = threadIdx.x; e < N * tile; e += NT) {
     const int n = e / tile, p2 = e % tile;
     base[(long)n * inner + p2] = sm[p2 * LD + n];
@@ -255,10 +259,94 @@ __global__ void k_set_int(int* p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-// Contraction data2_a(k) = sum_b B_ab(k) data_b(k) (G.c:4609-4629), in place
-// on W.  B comes from Bsym (STORED) or is rebuilt per k in FP64 from
-// B_ab = (s_a:e_b - u_a.G.u_b)/mu, u_a = s_a f.  quirk != 0 adds
-// data_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441).
+// Contraction at one k-point (G.c:4609-4629): out_a = sum_b B_ab(k) v_b,
+// written to out[a * ostride] (v stays in registers, so out may alias the
+// locations v was read from).  Stored mode reads B from Bsym.  Recompute
+// mode never forms B: with u_a = s_a f (s_a = C:e_a) and w = sum_b u_b v_b,
+//   sum_b B_ab v_b = sum_b (s_a:e_b / mu) v_b - (G u_a) . w / mu,
+// the same B_ab = (s_a:e_b - u_a.G.u_b)/mu of Bmatrix (G.c:386-402) with the
+// sums regrouped; u and G are evaluated in FP64 and rounded to T.
+// quirk != 0 adds v_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441).
+template <typename T, int SM>
+__device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec, long pt,
+                                               int S, int quirk, bool stored, const T* __restrict__ Bsym, long P,
+                                               cx<T>* out, long ostride) {
+  double f[3], g[6];
+  kvec(pt, ec, f);
+  if (!green6(f, ec, g)) {  // B(0) = 0 and DD(0) = 0
+#pragma unroll
+    for (int a = 0; a < SM; a++)
+      if (a < S) out[a * ostride] = {(T)0, (T)0};
+    return;
+  }
+  double tf[3] = {0, 0, 0};
+  if (quirk) {
+    const double* t9 = ec->T;
+    tf[0] = t9[0] * f[0] + t9[1] * f[1] + t9[2] * f[2];
+    tf[1] = t9[3] * f[0] + t9[4] * f[1] + t9[5] * f[2];
+    tf[2] = t9[6] * f[0] + t9[7] * f[1] + t9[8] * f[2];
+  }
+  cx<T> w0 = {(T)0, (T)0}, w1 = {(T)0, (T)0}, w2 = {(T)0, (T)0};
+  if (!stored) {
+#pragma unroll
+    for (int b = 0; b < SM; b++) {
+      if (b >= S) continue;
+      const double* s = ec->sig[b];
+      const T u0 = (T)(s[0] * f[0] + s[1] * f[1] + s[2] * f[2]);
+      const T u1 = (T)(s[3] * f[0] + s[4] * f[1] + s[5] * f[2]);
+      const T u2 = (T)(s[6] * f[0] + s[7] * f[1] + s[8] * f[2]);
+      w0.x += u0 * v[b].x; w0.y += u0 * v[b].y;
+      w1.x += u1 * v[b].x; w1.y += u1 * v[b].y;
+      w2.x += u2 * v[b].x; w2.y += u2 * v[b].y;
+    }
+  }
+  // the a loop stays rolled (register pressure); v is only indexed by the
+  // unrolled b loops, so it stays in registers
+#pragma unroll 1
+  for (int a = 0; a < S; a++) {
+    double gu[3] = {0, 0, 0};
+    if (!stored || quirk) {
+      const double* s = ec->sig[a];
+      const double u[3] = {s[0] * f[0] + s[1] * f[1] + s[2] * f[2], s[3] * f[0] + s[4] * f[1] + s[5] * f[2],
+                           s[6] * f[0] + s[7] * f[1] + s[8] * f[2]};
+      gmul(g, u, gu);
+    }
+    cx<T> acc = {(T)0, (T)0};
+    if (stored) {
+#pragma unroll
+      for (int b = 0; b < SM; b++) {
+        if (b >= S) continue;
+        const T bab = Bsym[(long)tri_index(a, b, S) * P + pt];
+        acc.x += v[b].x * bab;
+        acc.y += v[b].y * bab;
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < SM; b++) {
+        if (b >= S) continue;
+        const T se = (T)(ec->SE[a][b] * ec->inv_mu);
+        acc.x += v[b].x * se;
+        acc.y += v[b].y * se;
+      }
+      const T g0 = (T)(gu[0] * ec->inv_mu), g1 = (T)(gu[1] * ec->inv_mu), g2 = (T)(gu[2] * ec->inv_mu);
+      acc.x -= g0 * w0.x + g1 * w1.x + g2 * w2.x;
+      acc.y -= g0 * w0.y + g1 * w1.y + g2 * w2.y;
+    }
+    if (quirk) {
+      const T dd = (T)((ec->Tsum[a] - (gu[0] * tf[0] + gu[1] * tf[1] + gu[2] * tf[2])) * ec->inv_mu);
+      cx<T> va = v[0];
+#pragma unroll
+      for (int b = 1; b < SM; b++)
+        if (b == a) va = v[b];
+      acc.x += va.x * dd;
+      acc.y += va.y * dd;
+    }
+    out[a * ostride] = acc;
+  }
+}
+
+// Pointwise contraction in place on W (used when the fused column kernel
+// does not fit).
 template <typename T, int SM, bool STORED>
 __global__ void __launch_bounds__(256)
 k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConsts* __restrict__ ec,
@@ -270,53 +358,104 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
 #pragma unroll
   for (int b = 0; b < SM; b++)
     if (b < S) v[b] = W[(long)b * P + pt];
-  double f[3], g[6];
-  kvec(pt, ec, f);
-  const bool nz = green6(f, ec, g);
-  double u[SM][3];
-  if (!STORED || quirk) {
+  contract_point<T, SM>(v, ec, pt, S, quirk, STORED, Bsym, P, W + pt, P);
+}
+
+// ---------------------------------------------------------------- fused kernels
+// K1: one block owns `tile` rows (along N3) of one field.  Inverse transform
+// along N3 finishes data2 in real space (G.c:4638), then plasticevolv
+// (G.c:3470-3501) updates xi, the new xi is copied into the rows
+// (G.c:4581-4590) and transformed forward along N3 (first pass of
+// G.c:4599).  The block's sum of Re xi / P goes to partials[f][block].
+template <typename T, int N, bool MASK, bool TAUPP>
+__global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
+k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
+           const double* __restrict__ taupp, StepArgs a, int tile, const cx<T>* __restrict__ tw,
+           double* __restrict__ partials) {
+  constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), NT = TILE * TPP, LD = pitch<N>();
+  static_assert(NT % 64 == 0, "block must be whole waves");
+  __shared__ cx<T> sm[TILE * LD];
+  __shared__ double red[NT / 64];
+  const long P = a.P;
+  const long rows_per_field = P / N;
+  const long row0 = (long)blockIdx.x * tile;
+  const int f = (int)(row0 / rows_per_field);
+  const long pt0 = (row0 - (long)f * rows_per_field) * N;
+  const long off = (long)f * P + pt0;
+  cx<T>* wb = W + off;
+  double2* xb = xi + off;
+  for (int e = threadIdx.x; e < tile * N; e += NT) sm[(e / N) * LD + (e % N)] = wb[e];
+  __syncthreads();
+  fft_lds<T, N, LD>(sm, tile, (T)1, tw);
+  const double nsize = (double)P;
+  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f], tauf = a.tau[f];
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < tile * N; e += NT) {
+    const int li = (e / N) * LD + (e % N);
+    const cx<T> d = sm[li];
+    double2 x = xb[e];
+    if (!MASK || mask[off + e] == 0) {
+      const double t = TAUPP ? taupp[(pt0 + e) * a.ns + f] : tauf;
+      x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / dsl);
+      x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
+      xb[e] = x;
+    }
+    sm[li] = {(T)x.x, (T)x.y};
+    acc += x.x / nsize;
+  }
+  __syncthreads();
+  fft_lds<T, N, LD>(sm, tile, (T)-1, tw);
+  for (int e = threadIdx.x; e < tile * N; e += NT) wb[e] = sm[(e / N) * LD + (e % N)];
+  const double sum = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[(long)f * (rows_per_field / tile) + (pt0 / N) / tile] = sum;
+}
+
+// K3: one block owns `tile` consecutive (i2,i3) columns of every field.
+// Forward transform along N1 (last pass of G.c:4599), contraction with B(k)
+// at every k of the tile (G.c:4609-4629), inverse transform along N1 (first
+// pass of G.c:4638).  LDS holds S*tile pencils (dynamic, S*tile*LD complex).
+// B(k) is recomputed per k (the stored-B mode takes the unfused passes).
+constexpr int C3_THREADS = 1024;
+// Pencils one block can hold: one per TPP-thread group for FP64 (register
+// budget of 128 VGPRs at 1024 threads), two for FP32.
+template <typename T, int N> constexpr int c3_max_pencils() {
+  return (sizeof(T) == 8 ? 1 : 2) * (C3_THREADS / make_plan(N).tpp);
+}
+
+template <typename T, int N, int SM>
+__global__ void __launch_bounds__(C3_THREADS)
+k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int tile, int quirk,
+               const cx<T>* __restrict__ tw) {
+  constexpr int TPP = make_plan(N).tpp, LD = pitch<N>(), PPI = C3_THREADS / TPP;
+  constexpr int G = (c3_max_pencils<T, N>() + PPI - 1) / PPI;
+  extern __shared__ unsigned char lds_raw[];
+  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  const int S = ec->ns;
+  const long inner = P / N;
+  const long m0 = (long)blockIdx.x * tile;
+  const int npen = S * tile;
+  const int per_field = N * tile;
+  for (int e = threadIdx.x; e < S * per_field; e += C3_THREADS) {
+    const int fld = e / per_field, r = e - fld * per_field, n = r / tile, p = r - n * tile;
+    sm[(fld * tile + p) * LD + n] = W[(long)fld * P + (long)n * inner + m0 + p];
+  }
+  __syncthreads();
+  const int q = threadIdx.x % TPP, p0 = threadIdx.x / TPP;
+  fft_lds_multi<T, N, G, LD>(sm, p0, PPI, npen, q, (T)-1, tw);
+  for (int e = threadIdx.x; e < per_field; e += C3_THREADS) {
+    const int p = e / N, n = e - p * N;  // consecutive threads read consecutive LDS words
+    cx<T> v[SM];
 #pragma unroll
     for (int b = 0; b < SM; b++)
-      if (b < S) {
-        const double* s = ec->sig[b];
-        u[b][0] = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
-        u[b][1] = s[3] * f[0] + s[4] * f[1] + s[5] * f[2];
-        u[b][2] = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
-      }
+      if (b < S) v[b] = sm[(b * tile + p) * LD + n];
+    contract_point<T, SM>(v, ec, (long)n * inner + m0 + p, S, quirk, false, (const T*)nullptr, P, sm + p * LD + n,
+                          (long)tile * LD);
   }
-  double tf[3] = {0, 0, 0};
-  if (quirk) {
-    const double* t9 = ec->T;
-    tf[0] = t9[0] * f[0] + t9[1] * f[1] + t9[2] * f[2];
-    tf[1] = t9[3] * f[0] + t9[4] * f[1] + t9[5] * f[2];
-    tf[2] = t9[6] * f[0] + t9[7] * f[1] + t9[8] * f[2];
-  }
-#pragma unroll
-  for (int a = 0; a < SM; a++) {
-    if (a >= S) continue;
-    cx<T> acc = {(T)0, (T)0};
-    if (nz) {
-      double gu[3] = {0, 0, 0};
-      if (!STORED || quirk) gmul(g, u[a], gu);
-#pragma unroll
-      for (int b = 0; b < SM; b++) {
-        if (b >= S) continue;
-        T bab;
-        if (STORED) {
-          bab = Bsym[(long)tri_index(a, b, S) * P + pt];
-        } else {
-          bab = (T)((ec->SE[a][b] - (gu[0] * u[b][0] + gu[1] * u[b][1] + gu[2] * u[b][2])) * ec->inv_mu);
-        }
-        acc.x += v[b].x * bab;
-        acc.y += v[b].y * bab;
-      }
-      if (quirk) {
-        const T dd = (T)((ec->Tsum[a] - (gu[0] * tf[0] + gu[1] * tf[1] + gu[2] * tf[2])) * ec->inv_mu);
-        acc.x += v[a].x * dd;
-        acc.y += v[a].y * dd;
-      }
-    }
-    W[(long)a * P + pt] = acc;
+  __syncthreads();
+  fft_lds_multi<T, N, G, LD>(sm, p0, PPI, npen, q, (T)1, tw);
+  for (int e = threadIdx.x; e < S * per_field; e += C3_THREADS) {
+    const int fld = e / per_field, r = e - fld * per_field, n = r / tile, p = r - n * tile;
+    W[(long)fld * P + (long)n * inner + m0 + p] = sm[(fld * tile + p) * LD + n];
   }
 }
 

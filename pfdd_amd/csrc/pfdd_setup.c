@@ -111,3 +111,34 @@ long pfdd_add_loop(double *xi, int n1, int n2, int n3, int sys, const double nor
   }
   return count;
 }
+
+int pfdd_seed_loops(double *xi, int n1, int n2, int n3, int ns, const double (*normals)[3], int nloops,
+                    double rmin, double rmax, unsigned long long *rng_state, double *loops_out)
+{
+  if (!xi || !normals || !rng_state || ns < 1 || nloops < 0) return PFDD_E_SHAPE;
+  for (int l = 0; l < nloops; l++) {
+    double r = rmin + (rmax - rmin) * pfdd_uniform01(rng_state);
+    int a = (int)(pfdd_uniform01(rng_state) * ns);
+    if (a > ns - 1) a = ns - 1;
+    double c[3];
+    c[0] = pfdd_uniform01(rng_state) * n1;
+    c[1] = pfdd_uniform01(rng_state) * n2;
+    c[2] = pfdd_uniform01(rng_state) * n3;
+    long cnt = pfdd_add_loop(xi, n1, n2, n3, a, normals[a], c, r);
+    if (loops_out) {
+      double *o = loops_out + 6 * l;
+      o[0] = a; o[1] = c[0]; o[2] = c[1]; o[3] = c[2]; o[4] = r; o[5] = (double)cnt;
+    }
+  }
+  return PFDD_OK;
+}
+
+void pfdd_seed_sigma(double amp, unsigned long long *rng_state, double sigma[3][3])
+{
+  static const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  for (int t = 0; t < 6; t++) {
+    double v = -amp + 2 * amp * pfdd_uniform01(rng_state);
+    sigma[ij[t][0]][ij[t][1]] = v;
+    sigma[ij[t][1]][ij[t][0]] = v;
+  }
+}

@@ -1,0 +1,390 @@
+/*
+ * pfdd -- driver of the PFDD FFT-elasticity + Ginzburg-Landau loop on one
+ * MI355X through libpfdd_gpu.so.
+ *
+ * It restates the control flow of main (PFDD_Final_GPU.c:4087-4847) for the
+ * GL path with runtime parameters instead of compile-time #defines: outer
+ * steps it = 0..NT-1, strain/stress at the steps where calc_strain_stress
+ * runs (G.c:4044), the outer-step prologue with the virtualevolv data2 term
+ * (G.c:4450-4452, 1426-1447), and the GL loop (G.c:4565-4691) in fixed-step
+ * mode or with plasticconverge (G.c:3732-3785).  interfacevolv and the
+ * virtual-strain update are off (single material, xi_v = 0).  Log lines and
+ * the files stress-strain.dat, phasefield.dat, pfout.dat, xi.dat and
+ * xi_bc.dat keep the reference formats.
+ *
+ *   pfdd [preset] [key=value ...]
+ * presets: c0 (32^3 x 3, one loop, 500 steps), c1 (128^3 x 12, 16 loops,
+ * 1000 steps), c2 (256^3 x 12, 64 loops, 2000 steps, mixed precision).
+ * keys: n1 n2 n3 systems(plane111|fcc) nloops rmin rmax seed sigma_amp tau0
+ *       steps nt converge plastic_max precision(fp64|mixed) bmode(recompute|stored)
+ *       quirk pfout_every outdir
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "pfdd_gpu.h"
+
+#define NV 9 /* virtual-strain fields kept in the binary dumps (G.c:88) */
+
+static double wtime(void)
+{
+  struct timeval tv;
+  gettimeofday(&tv, NULL);
+  return tv.tv_sec + tv.tv_usec / 1e6;
+}
+
+typedef struct {
+  int n1, n2, n3;
+  char systems[16];     /* plane111: FCC systems 0-2 rotated so (111) = z; fcc: all 12 */
+  int nloops;           /* 0: one loop of radius rmin at the grid centre on system 0 */
+  double rmin, rmax;
+  unsigned long long seed;
+  double sigma_amp;     /* random symmetric applied stress amplitude (fcc) */
+  double tau0;          /* resolved shear on system 0 (plane111) */
+  int steps;            /* GL steps per plastic outer step (fixed-step mode) */
+  int nt;               /* outer steps NT (G.c:90); GL loops run for 0 < it < NT-1 */
+  int converge;         /* 1: stop on plasticconverge, at most plastic_max steps */
+  int plastic_max;
+  int precision, bmode, quirk;
+  int pfout_every;      /* 0: no pfout.dat */
+  char outdir[512];
+} config;
+
+static void preset(config *c, const char *name)
+{
+  memset(c, 0, sizeof(*c));
+  c->nt = 3;
+  c->plastic_max = 3000;
+  c->quirk = 1;
+  c->precision = PFDD_FP64;
+  c->bmode = PFDD_B_RECOMPUTE;
+  c->seed = 20261202ULL;
+  strcpy(c->outdir, ".");
+  if (!strcmp(name, "c0")) {
+    c->n1 = c->n2 = c->n3 = 32;
+    strcpy(c->systems, "plane111");
+    c->nloops = 0;
+    c->rmin = c->rmax = 8.0;
+    c->tau0 = 0.01;
+    c->steps = 500;
+  } else if (!strcmp(name, "c1") || !strcmp(name, "c2")) {
+    int big = !strcmp(name, "c2");
+    c->n1 = c->n2 = c->n3 = big ? 256 : 128;
+    strcpy(c->systems, "fcc");
+    c->nloops = big ? 64 : 16;
+    c->rmin = 5.0;
+    c->rmax = 20.0;
+    c->seed += big ? 2 : 1;
+    c->sigma_amp = 0.02;
+    c->steps = big ? 2000 : 1000;
+    if (big) c->precision = PFDD_MIXED;
+  } else {
+    fprintf(stderr, "pfdd: unknown preset %s\n", name);
+    exit(2);
+  }
+}
+
+static void set_key(config *c, const char *kv)
+{
+  char key[64];
+  const char *eq = strchr(kv, '=');
+  if (!eq || eq - kv >= (long)sizeof(key)) {
+    fprintf(stderr, "pfdd: expected key=value, got %s\n", kv);
+    exit(2);
+  }
+  memcpy(key, kv, eq - kv);
+  key[eq - kv] = 0;
+  const char *v = eq + 1;
+  if (!strcmp(key, "n1")) c->n1 = atoi(v);
+  else if (!strcmp(key, "n2")) c->n2 = atoi(v);
+  else if (!strcmp(key, "n3")) c->n3 = atoi(v);
+  else if (!strcmp(key, "systems")) snprintf(c->systems, sizeof(c->systems), "%s", v);
+  else if (!strcmp(key, "nloops")) c->nloops = atoi(v);
+  else if (!strcmp(key, "rmin")) c->rmin = atof(v);
+  else if (!strcmp(key, "rmax")) c->rmax = atof(v);
+  else if (!strcmp(key, "seed")) c->seed = strtoull(v, NULL, 10);
+  else if (!strcmp(key, "sigma_amp")) c->sigma_amp = atof(v);
+  else if (!strcmp(key, "tau0")) c->tau0 = atof(v);
+  else if (!strcmp(key, "steps")) c->steps = atoi(v);
+  else if (!strcmp(key, "nt")) c->nt = atoi(v);
+  else if (!strcmp(key, "converge")) c->converge = atoi(v);
+  else if (!strcmp(key, "plastic_max")) c->plastic_max = atoi(v);
+  else if (!strcmp(key, "precision")) c->precision = !strcmp(v, "mixed") ? PFDD_MIXED : PFDD_FP64;
+  else if (!strcmp(key, "bmode")) c->bmode = !strcmp(v, "stored") ? PFDD_B_STORED : PFDD_B_RECOMPUTE;
+  else if (!strcmp(key, "quirk")) c->quirk = atoi(v);
+  else if (!strcmp(key, "pfout_every")) c->pfout_every = atoi(v);
+  else if (!strcmp(key, "outdir")) snprintf(c->outdir, sizeof(c->outdir), "%s", v);
+  else {
+    fprintf(stderr, "pfdd: unknown key %s\n", key);
+    exit(2);
+  }
+}
+
+static FILE *open_out(const config *c, const char *name, const char *mode)
+{
+  char path[1024];
+  snprintf(path, sizeof(path), "%s/%s", c->outdir, name);
+  FILE *f = fopen(path, mode);
+  if (!f) {
+    perror(path);
+    exit(1);
+  }
+  return f;
+}
+
+static void die(int rc, const char *what)
+{
+  if (rc != PFDD_OK) {
+    fprintf(stderr, "pfdd: %s failed: %s (%d)\n", what, pfdd_strerror(rc), rc);
+    exit(1);
+  }
+}
+
+/* plasticconverge, G.c:3732-3785 */
+static int plastic_converge(double gamma, double gammalast, int it_plastic, int *testplastic, int *countgamma)
+{
+  int flag = 1;
+  int hit;
+  if (gamma != 0) {
+    double rel = fabs((gamma - gammalast) / gamma);
+    hit = rel < 1e-8 || fabs(gamma) < 1e-12;
+    if (hit) printf("(gamma-gammalast)/gamma = %e     %d\n", (gamma - gammalast) / gamma, *countgamma);
+  } else {
+    hit = fabs(gamma - gammalast) < 1e-15 || fabs(gamma) < 1e-14;
+    if (hit) printf("gamma-gammalast = %e     %d\n", gamma - gammalast, *countgamma);
+  }
+  if (hit) {
+    testplastic[it_plastic] = 1;
+    if (it_plastic >= 29) {
+      int sum = 0;
+      for (int i = 0; i < 30; i++) sum += testplastic[it_plastic - i];
+      if (sum == 30) (*countgamma)++;
+      else *countgamma = 0;
+    }
+  }
+  if (*countgamma == 5) {
+    (*countgamma)++;
+    flag = 0;
+  }
+  return flag;
+}
+
+/* S_ijkl:sigma*mu, the xi-independent part of avepst (avestrain G.c:656-729). */
+static void avepst_uniform(const pfdd_params *p, const double sigma[3][3], double out[9])
+{
+  double C11 = p->C11, C12 = p->C12, C44 = p->C44, mu = C44;
+  double S11 = 1. / 3. * (1 / (C11 + 2 * C12) + 2 / (C11 - C12));
+  double S12 = 1. / 3. * (1 / (C11 + 2 * C12) - 1 / (C11 - C12));
+  double S44 = 1 / C44;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0.0;
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) {
+          double dij = i == j, dkl = k == l, dik = i == k, djl = j == l, dil = i == l, djk = j == k;
+          double d4 = (i == j && j == k && k == l);
+          double S = (S11 + 2 * S12) / 3.0 * dij * dkl + S44 / 4.0 * (dik * djl + dil * djk) - S44 / 2.0 * d4 +
+                     (S11 - S12) * (d4 - dij * dkl / 3.0);
+          v += S * sigma[k][l] * mu;
+        }
+      out[3 * i + j] = v;
+    }
+}
+
+int main(int argc, char **argv)
+{
+  config cfg;
+  int argi = 1;
+  if (argc > 1 && !strchr(argv[1], '=')) preset(&cfg, argv[argi++]);
+  else preset(&cfg, "c0");
+  for (; argi < argc; argi++) set_key(&cfg, argv[argi]);
+
+  double tmstart = wtime();
+  const double nu = 0.33, mu = 1.0, lam = 2 * nu / (1 - 2 * nu);
+  pfdd_params p;
+  memset(&p, 0, sizeof(p));
+  p.n1 = cfg.n1; p.n2 = cfg.n2; p.n3 = cfg.n3;
+  p.precision = cfg.precision;
+  p.bmode = cfg.bmode;
+  p.C44 = mu; p.C12 = lam;
+  p.C11 = 2. * p.C44 + p.C12; /* MT == 1: isotropic (G.c:4243-4245) */
+  p.d1 = p.d2 = p.d3 = 1.0;
+  p.uq2 = 1.0;
+  p.pi_const = 3.141592654;
+  printf("This is an isotropic material.\n");
+  printf("Material constants for material 0\n");
+  printf("C11 = %e, C12 = %e, C44 = %e\n", p.C11, p.C12, p.C44);
+
+  /* slip systems and eigenstrains (setfcc G.c:1812, setorient G.c:1986, seteps G.c:1912) */
+  double xn12[12][3], xb12[12][3], theta[3][3], normals[PFDD_MAX_NS][3], dslip[PFDD_MAX_NS];
+  pfdd_setfcc(xn12, xb12);
+  int nsys = 0;
+  double rows[3][3];
+  if (!strcmp(cfg.systems, "plane111")) {
+    const double a = 1 / sqrt(2.), b = 1 / sqrt(6.), t = 1 / sqrt(3.);
+    double r[3][3] = {{-a, a, 0.0}, {-b, -b, 2 * b}, {t, t, t}};
+    memcpy(rows, r, sizeof(rows));
+    nsys = 3;
+  } else if (!strcmp(cfg.systems, "fcc")) {
+    double r[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    memcpy(rows, r, sizeof(rows));
+    nsys = 12;
+  } else {
+    fprintf(stderr, "pfdd: unknown systems %s\n", cfg.systems);
+    return 2;
+  }
+  pfdd_setorient(rows, theta);
+  p.ns = nsys;
+  for (int a = 0; a < nsys; a++) {
+    dslip[a] = 1.0;
+    p.cd2[a] = 0.1;                    /* G.c:4290 */
+    p.asf2[a] = 0.0262;                /* G.c:4279-4282 */
+    p.dslip[a] = 1.0;
+    for (int i = 0; i < 3; i++) {
+      normals[a][i] = 0.0;
+      for (int k = 0; k < 3; k++) normals[a][i] += theta[i][k] * xn12[a][k];
+    }
+  }
+  for (int a = 0; a < nsys && a < 2; a++) p.asf2[a] = 0.0064659518668288441;
+  pfdd_seteps(nsys, (const double(*)[3])xn12, (const double(*)[3])xb12, dslip, theta, p.eps);
+  printf("Slip systems %d\n", nsys);
+
+  /* initial xi (seeded loops) and applied stress */
+  const long P = (long)p.n1 * p.n2 * p.n3;
+  const int nsv = nsys + NV;
+  double *xi = calloc((size_t)2 * nsv * P, sizeof(double));
+  double *xi_bc = calloc((size_t)2 * nsv * P, sizeof(double));
+  if (!xi || !xi_bc) {
+    fprintf(stderr, "pfdd: host allocation failed\n");
+    return 1;
+  }
+  for (long q = (long)nsys * P; q < (long)nsv * P; q++) xi_bc[2 * q] = 1.0; /* virtual fields frozen */
+  printf(" Generating initial data\n");
+  double sigma[3][3] = {{0}};
+  unsigned long long rng = cfg.seed;
+  if (cfg.nloops == 0) {
+    double c[3] = {p.n1 / 2.0, p.n2 / 2.0, p.n3 / 2.0};
+    pfdd_add_loop(xi, p.n1, p.n2, p.n3, 0, normals[0], c, cfg.rmin);
+  } else {
+    die(pfdd_seed_loops(xi, p.n1, p.n2, p.n3, nsys, (const double(*)[3])normals, cfg.nloops, cfg.rmin, cfg.rmax,
+                        &rng, NULL), "pfdd_seed_loops");
+  }
+  if (!strcmp(cfg.systems, "plane111")) {
+    double se[3][3], se_e = 0.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) se[i][j] = 0.5 * (p.eps[0][i][j] + p.eps[0][j][i]);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) se_e += se[i][j] * p.eps[0][i][j];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) sigma[i][j] = cfg.tau0 * se[i][j] / se_e;
+  } else {
+    pfdd_seed_sigma(cfg.sigma_amp, &rng, sigma);
+  }
+  double tau[PFDD_MAX_NS];
+  pfdd_resolve_tau(nsys, (const double(*)[3])sigma, (const double(*)[3][3])p.eps, tau);
+  printf("Resolved shear stresses\n");
+
+  printf("Frequencies\n");
+  printf("Set interaction matrix\n");
+  pfdd_ctx *ctx = NULL;
+  die(pfdd_create(&p, &ctx), "pfdd_create");
+  die(pfdd_set_tau(ctx, tau, 0), "pfdd_set_tau");
+  die(pfdd_upload_xi(ctx, xi, xi_bc, nsv), "pfdd_upload_xi");
+
+  FILE *of2 = open_out(&cfg, "stress-strain.dat", "w");
+  FILE *ofpfv = open_out(&cfg, "phasefield.dat", "w");
+  FILE *ofpfout = cfg.pfout_every > 0 ? open_out(&cfg, "pfout.dat", "w") : NULL;
+  double ap_u[9];
+  avepst_uniform(&p, sigma, ap_u);
+  int maxsteps = cfg.converge ? cfg.plastic_max : cfg.steps;
+  double *gam = malloc(sizeof(double) * (maxsteps > 0 ? maxsteps : 1));
+  double *gam_a = malloc(sizeof(double) * nsys * (maxsteps > 0 ? maxsteps : 1));
+  int *testplastic = calloc(cfg.plastic_max > 0 ? cfg.plastic_max : 1, sizeof(int));
+  double gamma = 0.0, gammalast = 0.0, gamma1[PFDD_MAX_NS] = {0};
+  long total_steps = 0;
+  double gl_time = 0.0;
+
+  printf("start time step \n");
+  for (int it = 0; it < cfg.nt; it++) {
+    const int itp = 0, itp2 = 0;
+    printf("time step, %d %d %d\n", it, itp, itp2);
+    printf("xi = %e\n", 0.0); /* xi[2*NS*N1*N2*N3]: the first virtual field */
+    /* calc_strain_stress (G.c:4044) */
+    if (it == cfg.nt - 1 || (it != 0 && it != cfg.nt - 1)) {
+      double aveps[9], avesig[9];
+      die(pfdd_strain_stress(ctx, ap_u, aveps, avesig, NULL, NULL), "pfdd_strain_stress");
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) fprintf(of2, "%e %e ", aveps[3 * i + j], avesig[3 * i + j] / mu);
+    }
+    /* fft_forward + calc_data2_datag + virtualevolv data2 term */
+    die(pfdd_begin_outer(ctx, cfg.quirk), "pfdd_begin_outer");
+    printf("vflag %d, time %d \n", 0, it);
+    if (it != 0 && it != cfg.nt - 1) {
+      printf("here plastic!\n");
+      printf("evolve plastic then evolve interface\n");
+      int countgamma = 0, it_plastic = 0, check = 1;
+      memset(testplastic, 0, sizeof(int) * (cfg.plastic_max > 0 ? cfg.plastic_max : 1));
+      double t0 = wtime();
+      if (!cfg.converge) {
+        die(pfdd_gl_steps(ctx, cfg.steps, gam, gam_a), "pfdd_gl_steps");
+        gl_time += wtime() - t0;
+        for (int s = 0; s < cfg.steps; s++) {
+          gamma = gam[s];
+          printf("in evolve plastic:    %d    %d\n", s, 1);
+          fprintf(ofpfv, "%d       %d         %e           %e           %e\n", it, s + 1, gam_a[(long)s * nsys],
+                  nsys > 1 ? gam_a[(long)s * nsys + 1] : 0.0, gamma);
+        }
+        for (int a = 0; a < nsys; a++) gamma1[a] = cfg.steps > 0 ? gam_a[(long)(cfg.steps - 1) * nsys + a] : 0.0;
+        total_steps += cfg.steps;
+      } else {
+        do { /* G.c:4574-4691, one GL step per call */
+          gammalast = gamma;
+          die(pfdd_gl_steps(ctx, 1, gam, gam_a), "pfdd_gl_steps");
+          gamma = gam[0];
+          for (int a = 0; a < nsys; a++) gamma1[a] = gam_a[a];
+          check = plastic_converge(gamma, gammalast, it_plastic, testplastic, &countgamma);
+          printf("in evolve plastic:    %d    %d\n", it_plastic, check);
+          it_plastic++;
+          total_steps++;
+          fprintf(ofpfv, "%d       %d         %e           %e           %e\n", it, it_plastic, gamma1[0],
+                  nsys > 1 ? gamma1[1] : 0.0, gamma);
+          if (ofpfout && (it_plastic % cfg.pfout_every == 0 || it_plastic == cfg.plastic_max || !check)) {
+            die(pfdd_download_xi(ctx, xi, nsv), "pfdd_download_xi");
+            fprintf(ofpfout, "zone   I = %d K = %d\n", p.n3, p.n1);
+            for (int i = 0; i < p.n1; i++)
+              for (int k = 0; k < p.n3; k++) fprintf(ofpfout, "%d   %d   %lf\n", k, i, xi[2 * ((long)i * p.n2 * p.n3 + k)]);
+          }
+        } while (check && it_plastic < cfg.plastic_max);
+        gl_time += wtime() - t0;
+      }
+    }
+    if (it == cfg.nt - 1) {
+      for (int a = 0; a < nsys; a++) fprintf(of2, "\t%e", gamma1[a]);
+      fprintf(of2, " %e \n", gamma);
+    }
+  }
+  /* binary dumps xi.dat / xi_bc.dat, [NSV][N1][N2][N3][2] doubles (G.c:4777-4788) */
+  die(pfdd_download_xi(ctx, xi, nsv), "pfdd_download_xi");
+  FILE *fx = open_out(&cfg, "xi.dat", "wb");
+  FILE *fb = open_out(&cfg, "xi_bc.dat", "wb");
+  fwrite(xi, sizeof(double), (size_t)2 * nsv * P, fx);
+  fwrite(xi_bc, sizeof(double), (size_t)2 * nsv * P, fb);
+  fclose(fx);
+  fclose(fb);
+  fclose(of2);
+  fclose(ofpfv);
+  if (ofpfout) fclose(ofpfout);
+  pfdd_destroy(ctx);
+  if (gl_time > 0)
+    printf("pfdd: %ld GL steps in %.3f s (%.1f GL steps/s)\n", total_steps, gl_time, total_steps / gl_time);
+  printf("wall time %fs\n", wtime() - tmstart);
+  free(xi);
+  free(xi_bc);
+  free(gam);
+  free(gam_a);
+  free(testplastic);
+  return 0;
+}

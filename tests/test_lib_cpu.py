@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "pfdd_gpu.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(pfdd_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b((?:pfdd_|fft_)[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_function():
@@ -128,3 +128,22 @@ def test_workloads_are_reproducible():
     assert a.tau[0] == pytest.approx(0.01, abs=1e-15)
     w = C.small_fcc((16, 16, 16))
     assert w.ns == 12 and w.xi.real.sum() > 0
+
+
+def test_driver_rejects_bad_arguments():
+    import subprocess
+    exe = os.path.join(ROOT, "pfdd_amd", "pfdd")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    r = subprocess.run([exe, "nosuchpreset"], capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown preset" in r.stderr
+    r = subprocess.run([exe, "c0", "frobnicate=3"], capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown key" in r.stderr
+
+
+def test_fft_seam_requires_init():
+    L = pa.load()
+    L.pfdd_fft_host_release()
+    a = np.zeros(8, dtype=np.complex64)
+    L.fft_forward(a.ctypes.data, 1)
+    assert L.pfdd_fft_host_status() == pa.E_STATE
