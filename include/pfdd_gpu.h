@@ -58,7 +58,17 @@ typedef struct {
   double uq2;                  /* 1.0 (G.c:4195) */
   double pi_const;             /* 3.141592654 (G.c:95) */
   double eps[PFDD_MAX_NS][3][3]; /* eigenstrains e_a (seteps G.c:1912-1980) */
+  int plan;                    /* kernel plan: PFDD_PLAN_AUTO (0) or PFDD_PLAN_* flags */
 } pfdd_params;
+
+/* Kernel plan flags (no reference counterpart; results are the same to
+ * rounding for every plan).  AUTO picks the measured-fastest plan. */
+enum {
+  PFDD_PLAN_AUTO = 0,
+  PFDD_PLAN_UNFUSED_ROWS = 1,  /* plain N3 passes + pointwise update instead of the fused row kernel */
+  PFDD_PLAN_UNFUSED_COLS = 2,  /* plain N1 passes + pointwise contraction instead of the fused column kernel */
+  PFDD_PLAN_FUSED_COLS = 4     /* fused column kernel whenever its tile fits in LDS */
+};
 
 typedef struct pfdd_ctx pfdd_ctx;
 

@@ -18,6 +18,7 @@ OK, E_HIP, E_ALLOC, E_SHAPE, E_RADIX, E_NAN, E_UNSUPPORTED, E_STATE = 0, -1, -2,
 FP64, MIXED = 0, 1
 B_STORED, B_RECOMPUTE = 0, 1
 FORWARD, BACKWARD = -1, 1
+PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS = 0, 1, 2, 4
 
 EXPORTED = [
     "pfdd_create", "pfdd_destroy", "pfdd_set_tau", "pfdd_upload_xi", "pfdd_download_xi",
@@ -46,6 +47,7 @@ class Params(ctypes.Structure):
         ("dslip", ctypes.c_double * MAX_NS),
         ("uq2", ctypes.c_double), ("pi_const", ctypes.c_double),
         ("eps", ctypes.c_double * (MAX_NS * 9)),
+        ("plan", ctypes.c_int),
     ]
 
 
@@ -119,7 +121,7 @@ def _check(rc, what):
 
 
 def make_params(shape, eps, C11, C12, C44, cd2, asf2, dslip, precision=FP64, bmode=B_RECOMPUTE,
-                d=(1.0, 1.0, 1.0), uq2=1.0, pi_const=3.141592654):
+                d=(1.0, 1.0, 1.0), uq2=1.0, pi_const=3.141592654, plan=PLAN_AUTO):
     p = Params()
     p.n1, p.n2, p.n3 = (int(v) for v in shape)
     eps = np.asarray(eps, dtype=np.float64).reshape(-1, 9)
@@ -132,6 +134,7 @@ def make_params(shape, eps, C11, C12, C44, cd2, asf2, dslip, precision=FP64, bmo
         for t in range(9):
             p.eps[9 * a + t] = eps[a, t]
     p.uq2, p.pi_const = uq2, pi_const
+    p.plan = plan
     return p
 
 

@@ -113,11 +113,11 @@ class Workload:
     def P(self):
         return int(np.prod(self.shape))
 
-    def params(self, precision=None, bmode=None):
+    def params(self, precision=None, bmode=None, plan=0):
         ns = self.ns
         return make_params(self.shape, self.eps, C11, C12, C44, [0.1] * ns, asf2_reference(ns), [1.0] * ns,
                            precision=self.precision if precision is None else precision,
-                           bmode=self.bmode if bmode is None else bmode, pi_const=PI_REF)
+                           bmode=self.bmode if bmode is None else bmode, pi_const=PI_REF, plan=plan)
 
     def s_tensor_avepst(self):
         """S:sigma*mu, the xi-independent part of avepst (avestrain G.c:714-729)."""

@@ -492,7 +492,8 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     const long strain_nblk = (P + 255) / 256;
     // fused row kernel: rows per block = the largest power of two <= its
     // compile-time tile that divides the N1*N2 rows of a field
-    if (fused_len(p->n3)) {
+    if (p->plan & ~(PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS)) FAIL(PFDD_E_SHAPE);
+    if (fused_len(p->n3) && !(p->plan & PFDD_PLAN_UNFUSED_ROWS)) {
       const long rows = (long)p->n1 * p->n2;
       const int tmax = row_tile_max(p->n3, c->fp64);
       int t = 1;
@@ -501,13 +502,13 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
       c->row_nblk = (int)(rows / t);
     }
     // fused column kernel: S*tile pencils of N1 points in LDS
-    if (fused_len(p->n1) && !c->stored) {
+    if (fused_len(p->n1) && !c->stored && !(p->plan & PFDD_PLAN_UNFUSED_COLS)) {
       const size_t per_col = (size_t)S * (p->n1 + 1) * csz;
       const int maxpen = (c->fp64 ? 1 : 2) * (C3_THREADS / make_plan(p->n1).tpp);
       const long inner = (long)p->n2 * p->n3;
       int t = 0;
       for (int cand = 1; cand * S <= maxpen && cand * per_col <= C3_LDS_BUDGET && inner % cand == 0; cand *= 2) t = cand;
-      if (t >= C3_MIN_TILE) {
+      if (t >= ((p->plan & PFDD_PLAN_FUSED_COLS) ? 1 : C3_MIN_TILE)) {
         c->c3_tile = t;
         c->c3_lds = t * per_col;
       }

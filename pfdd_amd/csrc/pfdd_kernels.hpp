@@ -266,8 +266,49 @@ __global__ void k_set_int(int* p, int v) {
 //   sum_b B_ab v_b = sum_b (s_a:e_b / mu) v_b - (G u_a) . w / mu,
 // the same B_ab = (s_a:e_b - u_a.G.u_b)/mu of Bmatrix (G.c:386-402) with the
 // sums regrouped; u and G are evaluated in FP64 and rounded to T.
-// quirk != 0 adds v_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441).
+// quirk != 0 adds v_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441) in a
+// second pass over a.  UNROLL_A unrolls the loop over outputs (more
+// registers); rolled, v is still only indexed by unrolled loops, so it stays
+// in registers.
+__device__ __forceinline__ void sig_f(const ElasticConsts* __restrict__ ec, int a, const double f[3], double u[3]) {
+  const double* s = ec->sig[a];
+  u[0] = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
+  u[1] = s[3] * f[0] + s[4] * f[1] + s[5] * f[2];
+  u[2] = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
+}
+
 template <typename T, int SM>
+__device__ __forceinline__ cx<T> contract_row(int a, const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec,
+                                              long pt, int S, bool stored, const T* __restrict__ Bsym, long P,
+                                              const double f[3], const double g[6], cx<T> w0, cx<T> w1, cx<T> w2) {
+  cx<T> acc = {(T)0, (T)0};
+  if (stored) {
+#pragma unroll
+    for (int b = 0; b < SM; b++) {
+      if (b >= S) continue;
+      const T bab = Bsym[(long)tri_index(a, b, S) * P + pt];
+      acc.x += v[b].x * bab;
+      acc.y += v[b].y * bab;
+    }
+  } else {
+    double u[3], gu[3];
+    sig_f(ec, a, f, u);
+    gmul(g, u, gu);
+#pragma unroll
+    for (int b = 0; b < SM; b++) {
+      if (b >= S) continue;
+      const T se = (T)(ec->SE[a][b] * ec->inv_mu);
+      acc.x += v[b].x * se;
+      acc.y += v[b].y * se;
+    }
+    const T g0 = (T)(gu[0] * ec->inv_mu), g1 = (T)(gu[1] * ec->inv_mu), g2 = (T)(gu[2] * ec->inv_mu);
+    acc.x -= g0 * w0.x + g1 * w1.x + g2 * w2.x;
+    acc.y -= g0 * w0.y + g1 * w1.y + g2 * w2.y;
+  }
+  return acc;
+}
+
+template <typename T, int SM, bool UNROLL_A>
 __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec, long pt,
                                                int S, int quirk, bool stored, const T* __restrict__ Bsym, long P,
                                                cx<T>* out, long ostride) {
@@ -279,69 +320,46 @@ __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const Elast
       if (a < S) out[a * ostride] = {(T)0, (T)0};
     return;
   }
-  double tf[3] = {0, 0, 0};
-  if (quirk) {
-    const double* t9 = ec->T;
-    tf[0] = t9[0] * f[0] + t9[1] * f[1] + t9[2] * f[2];
-    tf[1] = t9[3] * f[0] + t9[4] * f[1] + t9[5] * f[2];
-    tf[2] = t9[6] * f[0] + t9[7] * f[1] + t9[8] * f[2];
-  }
   cx<T> w0 = {(T)0, (T)0}, w1 = {(T)0, (T)0}, w2 = {(T)0, (T)0};
   if (!stored) {
 #pragma unroll
     for (int b = 0; b < SM; b++) {
       if (b >= S) continue;
-      const double* s = ec->sig[b];
-      const T u0 = (T)(s[0] * f[0] + s[1] * f[1] + s[2] * f[2]);
-      const T u1 = (T)(s[3] * f[0] + s[4] * f[1] + s[5] * f[2]);
-      const T u2 = (T)(s[6] * f[0] + s[7] * f[1] + s[8] * f[2]);
+      double ud[3];
+      sig_f(ec, b, f, ud);
+      const T u0 = (T)ud[0], u1 = (T)ud[1], u2 = (T)ud[2];
       w0.x += u0 * v[b].x; w0.y += u0 * v[b].y;
       w1.x += u1 * v[b].x; w1.y += u1 * v[b].y;
       w2.x += u2 * v[b].x; w2.y += u2 * v[b].y;
     }
   }
-  // the a loop stays rolled (register pressure); v is only indexed by the
-  // unrolled b loops, so it stays in registers
+  if (UNROLL_A) {
+#pragma unroll
+    for (int a = 0; a < SM; a++)
+      if (a < S) out[a * ostride] = contract_row<T, SM>(a, v, ec, pt, S, stored, Bsym, P, f, g, w0, w1, w2);
+  } else {
 #pragma unroll 1
-  for (int a = 0; a < S; a++) {
-    double gu[3] = {0, 0, 0};
-    if (!stored || quirk) {
-      const double* s = ec->sig[a];
-      const double u[3] = {s[0] * f[0] + s[1] * f[1] + s[2] * f[2], s[3] * f[0] + s[4] * f[1] + s[5] * f[2],
-                           s[6] * f[0] + s[7] * f[1] + s[8] * f[2]};
+    for (int a = 0; a < S; a++)
+      out[a * ostride] = contract_row<T, SM>(a, v, ec, pt, S, stored, Bsym, P, f, g, w0, w1, w2);
+  }
+  if (quirk) {
+    double tf[3];
+    const double* t9 = ec->T;
+    tf[0] = t9[0] * f[0] + t9[1] * f[1] + t9[2] * f[2];
+    tf[1] = t9[3] * f[0] + t9[4] * f[1] + t9[5] * f[2];
+    tf[2] = t9[6] * f[0] + t9[7] * f[1] + t9[8] * f[2];
+#pragma unroll
+    for (int a = 0; a < SM; a++) {
+      if (a >= S) continue;
+      double u[3], gu[3];
+      sig_f(ec, a, f, u);
       gmul(g, u, gu);
-    }
-    cx<T> acc = {(T)0, (T)0};
-    if (stored) {
-#pragma unroll
-      for (int b = 0; b < SM; b++) {
-        if (b >= S) continue;
-        const T bab = Bsym[(long)tri_index(a, b, S) * P + pt];
-        acc.x += v[b].x * bab;
-        acc.y += v[b].y * bab;
-      }
-    } else {
-#pragma unroll
-      for (int b = 0; b < SM; b++) {
-        if (b >= S) continue;
-        const T se = (T)(ec->SE[a][b] * ec->inv_mu);
-        acc.x += v[b].x * se;
-        acc.y += v[b].y * se;
-      }
-      const T g0 = (T)(gu[0] * ec->inv_mu), g1 = (T)(gu[1] * ec->inv_mu), g2 = (T)(gu[2] * ec->inv_mu);
-      acc.x -= g0 * w0.x + g1 * w1.x + g2 * w2.x;
-      acc.y -= g0 * w0.y + g1 * w1.y + g2 * w2.y;
-    }
-    if (quirk) {
       const T dd = (T)((ec->Tsum[a] - (gu[0] * tf[0] + gu[1] * tf[1] + gu[2] * tf[2])) * ec->inv_mu);
-      cx<T> va = v[0];
-#pragma unroll
-      for (int b = 1; b < SM; b++)
-        if (b == a) va = v[b];
-      acc.x += va.x * dd;
-      acc.y += va.y * dd;
+      cx<T> o = out[a * ostride];
+      o.x += v[a].x * dd;
+      o.y += v[a].y * dd;
+      out[a * ostride] = o;
     }
-    out[a * ostride] = acc;
   }
 }
 
@@ -358,7 +376,7 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
 #pragma unroll
   for (int b = 0; b < SM; b++)
     if (b < S) v[b] = W[(long)b * P + pt];
-  contract_point<T, SM>(v, ec, pt, S, quirk, STORED, Bsym, P, W + pt, P);
+  contract_point<T, SM, true>(v, ec, pt, S, quirk, STORED, Bsym, P, W + pt, P);
 }
 
 // ---------------------------------------------------------------- fused kernels
@@ -448,7 +466,7 @@ k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
 #pragma unroll
     for (int b = 0; b < SM; b++)
       if (b < S) v[b] = sm[(b * tile + p) * LD + n];
-    contract_point<T, SM>(v, ec, (long)n * inner + m0 + p, S, quirk, false, (const T*)nullptr, P, sm + p * LD + n,
+    contract_point<T, SM, false>(v, ec, (long)n * inner + m0 + p, S, quirk, false, (const T*)nullptr, P, sm + p * LD + n,
                           (long)tile * LD);
   }
   __syncthreads();

@@ -382,3 +382,29 @@ def test_fft_host_seam_matches_reference_fourn():
             ref = R.fourn3d(x[b], sign)
             assert np.abs(a[b] - ref).max() <= 2e-5 * np.abs(ref).max()
     L.pfdd_fft_host_release()
+
+
+@pytest.mark.parametrize("plan", [pa.PLAN_UNFUSED_ROWS, pa.PLAN_UNFUSED_COLS, pa.PLAN_FUSED_COLS,
+                                  pa.PLAN_UNFUSED_ROWS | pa.PLAN_FUSED_COLS])
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+def test_kernel_plans_agree(plan, prec):
+    """Every kernel plan computes the same GL step (fused or unfused passes)."""
+    say(f"plan {plan} prec {prec}")
+    w = C.small_fcc((32, 32, 32), ns=12)
+
+    def run(pl):
+        with pa.Context(w.params(prec, pa.B_RECOMPUTE, pl)) as c:
+            c.set_tau(w.tau)
+            c.upload_xi(w.xi)
+            c.begin_outer(True)
+            c.gl_steps(20)
+            return c.download_xi(), c.step_plan()
+
+    x0, p0 = run(pa.PLAN_AUTO)
+    x1, p1 = run(plan)
+    tol = 1e-13 if prec == pa.FP64 else 2e-6
+    assert np.abs(x1 - x0).max() <= tol
+    if plan & pa.PLAN_UNFUSED_ROWS:
+        assert p1[pa.STAGE_FFT_AXIS3] == 2
+    if plan & pa.PLAN_FUSED_COLS:
+        assert p1[pa.STAGE_FFT_AXIS1] == 0
