@@ -32,6 +32,8 @@ STAGES = [("k_update", pa.STAGE_UPDATE, 1), ("k_fft_rows(N3)", pa.STAGE_FFT_AXIS
 def stage_bytes(stage, S, P, ntri, fp64, stored):
     """Algorithmic bytes of one launch: each input read once, each output written once."""
     c = 16 if fp64 else 8
+    if stage == pa.STAGE_FINISH:
+        return 0
     r = 8 if fp64 else 4
     if stage == pa.STAGE_UPDATE:
         return S * P * (c + 16 + 16 + c)          # read W, read xi, write xi, write W
@@ -42,6 +44,37 @@ def stage_bytes(stage, S, P, ntri, fp64, stored):
     if stage == pa.STAGE_COPY:
         return S * P * (16 + c)
     raise ValueError(stage)
+
+
+def rocprof_name(stage, plan, fp64, shape, S, stored):
+    """The kernel symbol rocprofv3 reports for a stage of this context."""
+    T = "double" if fp64 else "float"
+    n1, n2, n3 = shape
+    sm = 4 if S <= 4 else 8 if S <= 8 else 12 if S <= 12 else 16
+    if stage == pa.STAGE_UPDATE:
+        return (f"void pfdd::k_row_step<{T}, {n3}, false, false>" if plan[pa.STAGE_FFT_AXIS3] == 0
+                else f"void pfdd::k_update<{T}, false, false>")
+    if stage == pa.STAGE_FFT_AXIS3:
+        return f"void pfdd::k_fft_rows<{T}, {n3}>"
+    if stage == pa.STAGE_FFT_AXIS2:
+        return f"void pfdd::k_fft_cols<{T}, {n2}>"
+    if stage == pa.STAGE_FFT_AXIS1:
+        return f"void pfdd::k_fft_cols<{T}, {n1}>"
+    if stage == pa.STAGE_CONTRACT:
+        return (f"void pfdd::k_col_contract<{T}, {n1}, {sm}>" if plan[pa.STAGE_FFT_AXIS1] == 0
+                else f"void pfdd::k_contract<{T}, {sm}, {'true' if stored else 'false'}>")
+    return None
+
+
+def measured_traffic(name):
+    """HBM bytes per dispatch from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE runs, calibrated on the plain copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "c1_128cubed_ns12_pmc_traffic.json")
+    if not name or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        k = json.load(fh)["kernels"].get(name)
+    return None if k is None else int(round(k["traffic_bytes"]))
 
 
 def step_alg_bytes(S, P, fp64, stored):
@@ -75,7 +108,8 @@ def time_stages(ctx, reps=20):
         names[pa.STAGE_UPDATE] = "k_row_step(N3 inv + update + N3 fwd)"
     if plan[pa.STAGE_FFT_AXIS1] == 0:
         names[pa.STAGE_CONTRACT] = "k_col_contract(N1 fwd + B(k) + N1 inv)"
-    todo = [(names[st], st, plan[st]) for _, st, _ in STAGES] + [("k_xi_to_w(copy)", pa.STAGE_COPY, 0)]
+    todo = [(names[st], st, plan[st]) for _, st, _ in STAGES] + [("k_finish_sums", pa.STAGE_FINISH, 1),
+                                                                 ("k_xi_to_w(copy)", pa.STAGE_COPY, 0)]
     for name, st, per_step in todo:
         ctx.enqueue_stage(st, 2)
         e0, e1 = hip.Event(), hip.Event()
@@ -145,17 +179,20 @@ def main():
     ms64, wall64 = timed_steps(ctx, a.steps, a.warmup)
     xi64 = ctx.download_xi()
     stages = time_stages(ctx)
+    plan = ctx.step_plan()
     ctx.close()
     sps64 = a.steps / (ms64 / 1e3)
 
     # dominant kernel of the FP64 step and its roofline
-    dom = max(((n, st, k, ms) for n, (st, k, ms) in stages.items() if k > 0), key=lambda t: t[2] * t[3])
+    dom = max(((n, st, k, ms) for n, (st, k, ms) in stages.items() if k > 0 and st != pa.STAGE_FINISH),
+              key=lambda t: t[2] * t[3])
     dname, dst, _, dms = dom
     dbytes = stage_bytes(dst, S, P, ntri, True, stored)
     achieved = dbytes / (dms / 1e3) / 1e9
+    rname = rocprof_name(dst, plan, True, w.shape, S, stored)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dname,
-                "alg_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(rname), "kernel": dname,
+                "rocprof_kernel": rname, "alg_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5)}
     step_bytes = step_alg_bytes(S, P, True, stored)
     step_gbs = step_bytes * sps64 / 1e9
     kernels = {n: {"avg_ms": round(ms, 5), "launches_per_step": k,

@@ -137,17 +137,18 @@ long long pfdd_device_bytes(const pfdd_ctx *c);
  * rocprofv3 can time it in isolation.  It overwrites the xi / work-array
  * state (upload xi again afterwards).  Asynchronous. */
 enum {
-  PFDD_STAGE_UPDATE = 0,    /* plasticevolv + copy + gamma sums (fused: + N3 passes) */
+  PFDD_STAGE_UPDATE = 0,    /* plasticevolv + copy + per-block gamma sums (fused: + N3 passes) */
   PFDD_STAGE_FFT_AXIS3 = 1, /* one pass along N3 (contiguous) over all fields */
   PFDD_STAGE_FFT_AXIS2 = 2, /* one pass along N2 */
   PFDD_STAGE_FFT_AXIS1 = 3, /* one pass along N1 */
   PFDD_STAGE_CONTRACT = 4,  /* B(k) contraction (fused: + N1 passes) */
   PFDD_STAGE_STEP = 5,      /* one whole GL step */
-  PFDD_STAGE_COPY = 6       /* plain copy xi -> work array (bandwidth reference) */
+  PFDD_STAGE_COPY = 6,      /* plain copy xi -> work array (bandwidth reference) */
+  PFDD_STAGE_FINISH = 7     /* the one-block pass that turns block sums into gamma */
 };
 int pfdd_enqueue_stage(pfdd_ctx *c, int stage, int reps);
 
-/* Launches of each stage per GL step for this context (launches[7]):
+/* Launches of each stage per GL step for this context (launches[8]):
  * PFDD_STAGE_UPDATE and PFDD_STAGE_CONTRACT name the fused row and column
  * kernels when the grid allows them, and the passes they absorb count 0. */
 int pfdd_step_plan(const pfdd_ctx *c, int *launches);

@@ -28,7 +28,8 @@ EXPORTED = [
     "pfdd_step_plan", "pfdd_seed_loops", "pfdd_seed_sigma", "pfdd_fft_host_init", "fft_forward", "fft_backward",
     "pfdd_fft_host_status", "pfdd_fft_host_release",
 ]
-STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY = range(7)
+(STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
+ STAGE_FINISH) = range(8)
 
 
 class PfddError(RuntimeError):
@@ -222,7 +223,7 @@ class Context:
         _check(load().pfdd_enqueue_stage(self.h, stage, reps), "pfdd_enqueue_stage")
 
     def step_plan(self):
-        a = (ctypes.c_int * 7)()
+        a = (ctypes.c_int * 8)()
         _check(load().pfdd_step_plan(self.h, a), "pfdd_step_plan")
         return list(a)
 

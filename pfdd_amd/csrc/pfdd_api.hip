@@ -146,7 +146,7 @@ int contract(pfdd_ctx* c, int quirk) {
 }
 
 template <typename T>
-int update(pfdd_ctx* c) {
+int update(pfdd_ctx* c, bool finish = true) {
   dim3 grid((unsigned)c->upd_nblk, (unsigned)c->S);
   cx<T>* W = (cx<T>*)c->W;
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
@@ -158,8 +158,9 @@ int update(pfdd_ctx* c) {
   else if (tp) PFDD_UP(false, true);
   else PFDD_UP(false, false);
 #undef PFDD_UP
-  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->upd_nblk, c->hist,
-                     c->step_ctr);
+  if (finish)
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->upd_nblk,
+                       c->hist, c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -204,15 +205,16 @@ void k1_launch_n(pfdd_ctx* c) {
 }
 
 template <typename T>
-int k1_launch(pfdd_ctx* c) {
+int k1_launch(pfdd_ctx* c, bool finish = true) {
   switch (c->p.n3) {
 #define PFDD_L(NN) case NN: k1_launch_n<T, NN>(c); break;
     PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
 #undef PFDD_L
     default: return PFDD_E_RADIX;
   }
-  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->row_nblk, c->hist,
-                     c->step_ctr);
+  if (finish)
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->row_nblk,
+                       c->hist, c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -775,8 +777,12 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
     const int sign = (r & 1) ? PFDD_BACKWARD : PFDD_FORWARD;
     switch (stage) {
       case PFDD_STAGE_UPDATE:
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
-        rc = c->row_tile > 0 ? k1_launch<T>(c) : update<T>(c);
+        rc = c->row_tile > 0 ? k1_launch<T>(c, false) : update<T>(c, false);
+        break;
+      case PFDD_STAGE_FINISH:
+        hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S,
+                           c->row_tile > 0 ? c->row_nblk : c->upd_nblk, c->hist, (int*)nullptr);
+        rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
         break;
       case PFDD_STAGE_FFT_AXIS3:
         rc = launch_axis<T>(W, (long)S * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream);
@@ -817,11 +823,12 @@ extern "C" int pfdd_step_plan(const pfdd_ctx* c, int* launches) {
   launches[PFDD_STAGE_CONTRACT] = 1;
   launches[PFDD_STAGE_STEP] = 0;
   launches[PFDD_STAGE_COPY] = 0;
+  launches[PFDD_STAGE_FINISH] = 1;
   return PFDD_OK;
 }
 
 extern "C" int pfdd_enqueue_stage(pfdd_ctx* c, int stage, int reps) {
-  if (!c || reps < 0 || stage < PFDD_STAGE_UPDATE || stage > PFDD_STAGE_COPY) return PFDD_E_SHAPE;
+  if (!c || reps < 0 || stage < PFDD_STAGE_UPDATE || stage > PFDD_STAGE_FINISH) return PFDD_E_SHAPE;
   c->prologue_done = 0;
   return c->fp64 ? enqueue_stage<double>(c, stage, reps) : enqueue_stage<float>(c, stage, reps);
 }

@@ -171,27 +171,33 @@ struct StepArgs {
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 
-// Second pass of the per-field sums: one block of FIN_THREADS threads adds
-// partials[f][0..nblk) for every field in a fixed order and writes out[f] and
-// out[ns] = sum over fields; with step_ctr the row is out + (*step_ctr)(ns+1)
-// and the counter advances (one row per GL step).
-constexpr int FIN_THREADS = 1024;
+// Second pass of the per-field sums: wave f of the single block adds
+// partials[f][0..nblk) in a fixed order (lane-strided, then a butterfly over
+// the wave), so no block-wide barrier sits inside the loop; then thread 0
+// adds the fields in order.  Writes out[f] and out[ns] = sum over fields;
+// with step_ctr the row is out + (*step_ctr)(ns+1) and the counter advances
+// (one row per GL step).  ns <= FIN_MAX_FIELDS.
+constexpr int FIN_THREADS = 1024, FIN_MAX_FIELDS = 32;
 __global__ void __launch_bounds__(FIN_THREADS)
 k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out, int* step_ctr) {
-  __shared__ double red[FIN_THREADS / 64];
-  double* dst = out;
-  if (step_ctr) dst = out + (long)(*step_ctr) * (ns + 1);
-  double tot = 0.0;
-  for (int f = 0; f < ns; f++) {
+  __shared__ double fsum[FIN_MAX_FIELDS];
+  const int lane = threadIdx.x & 63;
+  for (int f = threadIdx.x >> 6; f < ns; f += FIN_THREADS / 64) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += FIN_THREADS) v += partials[(long)f * nblk + b];
-    const double s = block_sum(v, red);
-    if (threadIdx.x == 0) {
-      dst[f] = s;
-      tot += s;
-    }
+    for (int b = lane; b < nblk; b += 64) v += partials[(long)f * nblk + b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) fsum[f] = v;
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double* dst = out;
+    if (step_ctr) dst = out + (long)(*step_ctr) * (ns + 1);
+    double tot = 0.0;
+    for (int g = 0; g < ns; g++) {
+      dst[g] = fsum[g];
+      tot += fsum[g];
+    }
     dst[ns] = tot;
     if (step_ctr) *step_ctr = *step_ctr + 1;
   }
