@@ -82,6 +82,12 @@ void pfdd_destroy(pfdd_ctx *c);
  * or the reference array tau[N1][N2][N3][NS] when per_point == 1. */
 int pfdd_set_tau(pfdd_ctx *c, const double *tau, int per_point);
 
+/* Linear ramp of the uniform resolved shear: the s-th GL step after this
+ * call (s = 0, 1, ...) uses tau_a(s) = tau0_a + rate_a * s, as if the
+ * applied stress of setstress/resolSS (G.c:3820, 1607) were raised between
+ * steps.  pfdd_set_tau ends the ramp. */
+int pfdd_set_tau_ramp(pfdd_ctx *c, const double *tau0, const double *rate);
+
 /* Copy the order parameter in.  xi and xi_bc are the reference arrays
  * [nsv][N1][N2][N3][2] with nsv >= ns (G.c:4146-4147); fields ns..nsv-1 are
  * the virtual strains and must be zero.  Only the real slot of xi_bc is read

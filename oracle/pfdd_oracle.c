@@ -382,6 +382,8 @@ struct orc_state {
   int ntri;
   double *xi, *bc, *tau, *data, *data2, *Bsym, *dds;
   int tau_pp;
+  double tau_rate[64];  /* uniform-tau ramp: tau_a(s) = tau_a + tau_rate_a * s */
+  long ramp_step;
 };
 
 static void k_of(const orc_state *s, long pt, double f[3])
@@ -460,6 +462,14 @@ void orc_set_tau(orc_state *s, const double *tau, int per_point)
   s->tau = (double *)malloc(sizeof(double) * n);
   memcpy(s->tau, tau, sizeof(double) * n);
   s->tau_pp = per_point;
+  memset(s->tau_rate, 0, sizeof(s->tau_rate));
+  s->ramp_step = 0;
+}
+
+void orc_set_tau_ramp(orc_state *s, const double *tau0, const double *rate)
+{
+  orc_set_tau(s, tau0, 0);
+  for (int a = 0; a < s->p.ns; a++) s->tau_rate[a] = rate[a];
 }
 
 /* data2_a = sum_b B_ab data_b, b ascending (G.c:4609-4629); quirk adds
@@ -550,14 +560,19 @@ void orc_gl_steps(orc_state *s, int nsteps, double *gamma, double *gamma_a)
     bc2 = (double *)calloc((size_t)2 * S * s->P, sizeof(double));
     for (long q = 0; q < S * s->P; q++) bc2[2 * q] = s->bc[q];
   }
+  double *tau_eff = (double *)malloc(sizeof(double) * (s->tau_pp ? (size_t)S * s->P : (size_t)S));
   for (int it = 0; it < nsteps; it++) {
     double ga[64];
+    const long n = s->tau_pp ? (long)S * s->P : S;
+    for (long q = 0; q < n; q++) tau_eff[q] = s->tau[q] + s->tau_rate[q % S] * (double)s->ramp_step;
+    s->ramp_step++;
     double g = orc_plasticevolv(S, s->P, s->xi, bc2, s->data2, s->p.cd2, s->p.asf2, s->p.dslip,
-                                s->tau, s->tau_pp, s->p.uq2, s->p.pi_const, ga);
+                                tau_eff, s->tau_pp, s->p.uq2, s->p.pi_const, ga);
     if (gamma) gamma[it] = g;
     if (gamma_a) memcpy(gamma_a + (size_t)it * S, ga, sizeof(double) * S);
     elastic_solve(s, 0);
   }
+  free(tau_eff);
   free(bc2);
 }
 

@@ -77,6 +77,8 @@ void orc_destroy(orc_state *s);
 void orc_set_xi(orc_state *s, const double *xi, const double *xi_bc);
 void orc_get_xi(const orc_state *s, double *xi);
 void orc_set_tau(orc_state *s, const double *tau, int per_point);
+/* Uniform-tau ramp: the s-th GL step after the call uses tau0_a + rate_a * s. */
+void orc_set_tau_ramp(orc_state *s, const double *tau0, const double *rate);
 
 /* Outer-step prologue (G.c:4450-4452 + virtualevolv G.c:1426-1447):
  * data = FFT(xi); data2 = B.data (+ data_b * sum DD_b when quirk);

@@ -26,7 +26,7 @@ EXPORTED = [
     "pfdd_device_bytes", "pfdd_strerror", "pfdd_setfcc", "pfdd_setorient", "pfdd_seteps",
     "pfdd_resolve_tau", "pfdd_splitmix64", "pfdd_uniform01", "pfdd_add_loop", "pfdd_enqueue_stage",
     "pfdd_step_plan", "pfdd_seed_loops", "pfdd_seed_sigma", "pfdd_fft_host_init", "fft_forward", "fft_backward",
-    "pfdd_fft_host_status", "pfdd_fft_host_release",
+    "pfdd_fft_host_status", "pfdd_fft_host_release", "pfdd_set_tau_ramp",
 ]
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
  STAGE_FINISH) = range(8)
@@ -68,6 +68,7 @@ def load():
     L.pfdd_destroy.argtypes = [vp]
     L.pfdd_destroy.restype = None
     L.pfdd_set_tau.argtypes = [vp, vp, ci]
+    L.pfdd_set_tau_ramp.argtypes = [vp, vp, vp]
     L.pfdd_upload_xi.argtypes = [vp, vp, vp, ci]
     L.pfdd_download_xi.argtypes = [vp, vp, ci]
     L.pfdd_begin_outer.argtypes = [vp, ci]
@@ -172,6 +173,11 @@ class Context:
     def set_tau(self, tau, per_point=False):
         t = np.ascontiguousarray(tau, dtype=np.float64)
         _check(load().pfdd_set_tau(self.h, t.ctypes.data, 1 if per_point else 0), "pfdd_set_tau")
+
+    def set_tau_ramp(self, tau0, rate):
+        t = np.ascontiguousarray(tau0, dtype=np.float64)
+        r = np.ascontiguousarray(rate, dtype=np.float64)
+        _check(load().pfdd_set_tau_ramp(self.h, t.ctypes.data, r.ctypes.data), "pfdd_set_tau_ramp")
 
     def upload_xi(self, xi, xi_bc=None):
         """xi: complex [nsv, N1, N2, N3] (nsv >= ns; fields past ns must be zero)."""

@@ -119,8 +119,9 @@ class Workload:
                            precision=self.precision if precision is None else precision,
                            bmode=self.bmode if bmode is None else bmode, pi_const=PI_REF, plan=plan)
 
-    def s_tensor_avepst(self):
+    def s_tensor_avepst(self, sigma=None):
         """S:sigma*mu, the xi-independent part of avepst (avestrain G.c:714-729)."""
+        sig = self.sigma if sigma is None else sigma
         S11 = 1. / 3. * (1 / (C11 + 2 * C12) + 2 / (C11 - C12))
         S12 = 1. / 3. * (1 / (C11 + 2 * C12) - 1 / (C11 - C12))
         S44 = 1 / C44
@@ -135,7 +136,7 @@ class Workload:
                         Sijkl = ((S11 + 2 * S12) / 3.0 * d[i, j] * d[k, l]
                                  + S44 / 4.0 * (d[i, k] * d[j, l] + d[i, l] * d[j, k])
                                  - S44 / 2.0 * d4 + (S11 - S12) * (d4 - d[i, j] * d[k, l] / 3.0))
-                        v += Sijkl * self.sigma[k, l] * MU
+                        v += Sijkl * sig[k, l] * MU
                 out[i, j] = v
         return out
 
@@ -204,6 +205,26 @@ def c2(bmode=B_STORED):
     return _fcc_random("c2", (256, 256, 256), 64, BASE_SEED + 2, 2000, MIXED, bmode)
 
 
+def c4(bmode=B_RECOMPUTE, shape=(512, 512, 512), steps=5000):
+    """512^3, NS=3: the three systems of one (111) plane (rotated so the plane
+    is z = const), 8 seeded loops r ~ U[20,60]; the applied stress
+    sigma = s_hat * 0.03 * t/(steps-1), s_hat = sym(e_0)/(sym(e_0):e_0), ramps the
+    resolved shear of system 0 linearly from 0 to 0.03 mu over the run.
+    Workload.tau holds tau(0) = 0 and Workload.tau_rate the per-step increment."""
+    theta = setorient(ROWS_111_TO_Z)
+    eps, normals = _fcc_systems([0, 1, 2], theta)
+    xi = np.zeros((3,) + tuple(shape), dtype=np.complex128)
+    rng = Rng(BASE_SEED + 4)
+    loops = _random_loops(rng, xi, normals, 8, 20.0, 60.0)
+    e0 = eps[0]
+    se = 0.5 * (e0 + e0.T)
+    s_hat = se / np.sum(se * e0)
+    w = Workload("c4", tuple(shape), eps, normals, xi, np.zeros(3), 0.03 * s_hat, steps, MIXED, bmode, loops)
+    w.sigma_unit = s_hat
+    w.tau_rate = resolve_tau(s_hat, eps) * 0.03 / (steps - 1)
+    return w
+
+
 def small_fcc(shape, nloops=4, seed=BASE_SEED + 100, ns=12, rmin=2.0, rmax=6.0):
     """A reduced FCC instance with the c1 generator, for parity tests."""
     theta = setorient(np.eye(3))
@@ -215,4 +236,4 @@ def small_fcc(shape, nloops=4, seed=BASE_SEED + 100, ns=12, rmin=2.0, rmax=6.0):
     return Workload(f"fcc{shape}", tuple(shape), eps, normals, xi, resolve_tau(sigma, eps), sigma, 10, loops=loops)
 
 
-BY_NAME = {"c0": c0, "c1": c1, "c2": c2}
+BY_NAME = {"c0": c0, "c1": c1, "c2": c2, "c4": c4}

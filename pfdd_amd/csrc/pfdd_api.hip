@@ -152,7 +152,7 @@ int update(pfdd_ctx* c, bool finish = true) {
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
 #define PFDD_UP(MM, TT)                                                                                  \
   hipLaunchKernelGGL((k_update<T, MM, TT>), grid, dim3(UPD_THREADS), 0, c->stream, c->xi, W, c->mask, c->taupp, \
-                     c->sa, c->partials)
+                     c->sa, c->partials, c->step_ctr)
   if (m && tp) PFDD_UP(true, true);
   else if (m) PFDD_UP(true, false);
   else if (tp) PFDD_UP(false, true);
@@ -195,7 +195,7 @@ void k1_launch_n(pfdd_ctx* c) {
   const cx<T>* tw = tw_of<T>(c->tw[2]);
 #define PFDD_K1(MM, TT)                                                                                   \
   hipLaunchKernelGGL((k_row_step<T, N, MM, TT>), dim3((unsigned)nb), dim3(NT), 0, c->stream, W, c->xi, c->mask, \
-                     c->taupp, c->sa, c->row_tile, tw, c->partials)
+                     c->taupp, c->sa, c->row_tile, tw, c->partials, c->step_ctr)
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
   if (m && tp) PFDD_K1(true, true);
   else if (m) PFDD_K1(true, false);
@@ -603,9 +603,22 @@ fail:
   return rc;
 }
 
+int pfdd_set_tau_ramp(pfdd_ctx* c, const double* tau0, const double* rate) {
+  if (!c || !tau0 || !rate) return PFDD_E_SHAPE;
+  for (int a = 0; a < c->S; a++)
+    if (!std::isfinite(tau0[a]) || !std::isfinite(rate[a])) return PFDD_E_NAN;
+  int rc = pfdd_set_tau(c, tau0, 0);
+  if (rc) return rc;
+  for (int a = 0; a < c->S; a++) c->sa.tau_rate[a] = rate[a];
+  return PFDD_OK;
+}
+
 int pfdd_set_tau(pfdd_ctx* c, const double* tau, int per_point) {
   if (!c || !tau) return PFDD_E_SHAPE;
   const int S = c->S;
+  for (int a = 0; a < S; a++) c->sa.tau_rate[a] = 0.0;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr + 1, 0);
+  HIPCK(hipPeekAtLastError());
   if (!per_point) {
     for (int a = 0; a < S; a++) {
       if (!std::isfinite(tau[a])) return PFDD_E_NAN;

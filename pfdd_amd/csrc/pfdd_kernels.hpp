@@ -167,6 +167,7 @@ struct StepArgs {
   double invP_unused;
   double uq2, pi;
   double cd2[MAXS], asf2[MAXS], dslip[MAXS], tau[MAXS];
+  double tau_rate[MAXS];   // tau_a(s) = tau_a + tau_rate_a * s, s = GL steps since the ramp was set
 };
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
@@ -175,8 +176,9 @@ constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 // partials[f][0..nblk) in a fixed order (lane-strided, then a butterfly over
 // the wave), so no block-wide barrier sits inside the loop; then thread 0
 // adds the fields in order.  Writes out[f] and out[ns] = sum over fields;
-// with step_ctr the row is out + (*step_ctr)(ns+1) and the counter advances
-// (one row per GL step).  ns <= FIN_MAX_FIELDS.
+// with step_ctr the row is out + step_ctr[0](ns+1) and both counters advance
+// (step_ctr[0]: history row, step_ctr[1]: GL steps under the current tau
+// ramp).  ns <= FIN_MAX_FIELDS.
 constexpr int FIN_THREADS = 1024, FIN_MAX_FIELDS = 32;
 __global__ void __launch_bounds__(FIN_THREADS)
 k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out, int* step_ctr) {
@@ -199,7 +201,10 @@ k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out
       tot += fsum[g];
     }
     dst[ns] = tot;
-    if (step_ctr) *step_ctr = *step_ctr + 1;
+    if (step_ctr) {
+      step_ctr[0] = step_ctr[0] + 1;  // history row
+      step_ctr[1] = step_ctr[1] + 1;  // GL steps since the tau ramp was set
+    }
   }
 }
 
@@ -208,14 +213,17 @@ k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out
 template <typename T, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__(UPD_THREADS)
 k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
-         const double* __restrict__ taupp, StepArgs a, double* __restrict__ partials) {
+         const double* __restrict__ taupp, StepArgs a, double* __restrict__ partials,
+         const int* __restrict__ step_ctr) {
   __shared__ double red[UPD_THREADS / 64];
   const int f = blockIdx.y;
   const long P = a.P;
   const double nsize = (double)P;
   const long off = (long)f * P;
   const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
-  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f], tauf = a.tau[f];
+  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
+  const double rate = a.tau_rate[f] * (double)step_ctr[1];
+  const double tauf = a.tau[f] + rate;
   double acc = 0.0;
 #pragma unroll
   for (int e = 0; e < UPD_EPT; e++) {
@@ -224,7 +232,7 @@ k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restr
       double2 x = xi[off + pt];
       const cx<T> d = W[off + pt];
       if (!MASK || mask[off + pt] == 0) {
-        const double t = TAUPP ? taupp[pt * a.ns + f] : tauf;
+        const double t = TAUPP ? taupp[pt * a.ns + f] + rate : tauf;
         x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / dsl);
         x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
         xi[off + pt] = x;
@@ -395,7 +403,7 @@ template <typename T, int N, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
 k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
            const double* __restrict__ taupp, StepArgs a, int tile, const cx<T>* __restrict__ tw,
-           double* __restrict__ partials) {
+           double* __restrict__ partials, const int* __restrict__ step_ctr) {
   constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), NT = TILE * TPP, LD = pitch<N>();
   static_assert(NT % 64 == 0, "block must be whole waves");
   __shared__ cx<T> sm[TILE * LD];
@@ -412,14 +420,16 @@ k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __res
   __syncthreads();
   fft_lds<T, N, LD>(sm, tile, (T)1, tw);
   const double nsize = (double)P;
-  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f], tauf = a.tau[f];
+  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
+  const double rate = a.tau_rate[f] * (double)step_ctr[1];
+  const double tauf = a.tau[f] + rate;
   double acc = 0.0;
   for (int e = threadIdx.x; e < tile * N; e += NT) {
     const int li = (e / N) * LD + (e % N);
     const cx<T> d = sm[li];
     double2 x = xb[e];
     if (!MASK || mask[off + e] == 0) {
-      const double t = TAUPP ? taupp[(pt0 + e) * a.ns + f] : tauf;
+      const double t = TAUPP ? taupp[(pt0 + e) * a.ns + f] + rate : tauf;
       x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / dsl);
       x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
       xb[e] = x;

@@ -1,0 +1,148 @@
+"""Run a benchmark configuration on one MI355X and report throughput and the
+mixed-precision error against the FP64 run of the same instance.
+
+    python -m pfdd_amd.run_config c2 [--steps N] [--json out.json]
+    python -m pfdd_amd.run_config c4 [--steps N] [--every 50] [--json out.json]
+
+c2: 256^3 x 12, 64 seeded loops: mixed and FP64 GL runs of N steps (default
+    2000); reports GL steps/s, grid-points x systems / s, and the errors of the
+    mixed run in xi (max abs), slipped area (sum_a gamma_a P) and mean stress.
+c4: 512^3 x 3 with the applied-shear ramp 0 -> 0.03 mu: mixed (recompute B)
+    and FP64 runs with <sigma>, <eps> sampled every `every` steps (the
+    stress-strain curve), plus mixed with stored B over a shorter window for
+    the stored-vs-recompute comparison.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import B_RECOMPUTE, B_STORED, FP64, MIXED, Context
+from . import configs as C
+from . import hip
+
+
+def _timed(ctx, n, record=True):
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(ctx.stream)
+    g, ga = ctx.gl_steps(n, record=record)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1), g, ga
+
+
+def run_plain(w, steps, precision, bmode, chunk=500):
+    ctx = Context(w.params(precision, bmode))
+    ctx.set_tau(w.tau)
+    ctx.upload_xi(w.xi)
+    ctx.begin_outer(True)
+    ms = 0.0
+    ga_all = []
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        t, _, ga = _timed(ctx, n)
+        ms += t
+        ga_all.append(ga)
+        done += n
+    ap = w.s_tensor_avepst()
+    aveps, avesig = ctx.strain_stress(ap)
+    xi = ctx.download_xi()
+    out = {"steps": steps, "ms": ms, "steps_per_s": steps / (ms / 1e3), "device_bytes": ctx.device_bytes}
+    ctx.close()
+    return out, xi, np.concatenate(ga_all), aveps, avesig
+
+
+def run_ramp(w, steps, precision, bmode, every):
+    ctx = Context(w.params(precision, bmode))
+    ctx.set_tau_ramp(w.tau, w.tau_rate)
+    ctx.upload_xi(w.xi)
+    ctx.begin_outer(True)
+    ms = 0.0
+    curve = []
+    done = 0
+    while done < steps:
+        n = min(every, steps - done)
+        t, _, ga = _timed(ctx, n)
+        ms += t
+        done += n
+        sig = w.sigma_unit * 0.03 * (done - 1) / (steps - 1)   # stress of the last step taken
+        aveps, avesig = ctx.strain_stress(w.s_tensor_avepst(sig))
+        ctx.begin_outer(False)                                 # rebuild data2 from the current xi
+        curve.append({"step": done, "applied_tau0": float(np.sum(sig * w.eps[0])),
+                      "gamma_a": ga[-1].tolist(), "aveps": aveps.tolist(), "avesigma": avesig.tolist()})
+    xi = ctx.download_xi()
+    out = {"steps": steps, "ms_gl_only": ms, "steps_per_s": steps / (ms / 1e3), "device_bytes": ctx.device_bytes}
+    ctx.close()
+    return out, xi, curve
+
+
+def step_bytes(S, P, fp64, stored):
+    if fp64:
+        return (144 * S + (4 * S * (S + 1) if stored else 0)) * P
+    return (96 * S + (2 * S * (S + 1) if stored else 0)) * P
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config", choices=["c2", "c4"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--every", type=int, default=50)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args()
+    t0 = time.time()
+    res = {"config": a.config}
+    if a.config == "c2":
+        w = C.c2()
+        steps = a.steps or w.steps
+        print(f"[run_config] c2 {w.shape} x {w.ns}, {steps} steps, setup {time.time() - t0:.1f}s", flush=True)
+        r64, x64, ga64, e64, s64 = run_plain(w, steps, FP64, B_RECOMPUTE)
+        print(f"[run_config] fp64 {r64['steps_per_s']:.2f} steps/s", flush=True)
+        r32, x32, ga32, e32, s32 = run_plain(w, steps, MIXED, B_STORED)
+        print(f"[run_config] mixed {r32['steps_per_s']:.2f} steps/s", flush=True)
+        area64, area32 = ga64[-1].sum() * w.P, ga32[-1].sum() * w.P
+        res.update({"shape": w.shape, "ns": w.ns, "steps": steps, "fp64": r64, "mixed_stored_B": r32,
+                    "speedup": r32["steps_per_s"] / r64["steps_per_s"],
+                    "gridpoints_x_systems_per_s_mixed": w.P * w.ns * r32["steps_per_s"],
+                    "xi_max_abs_err": float(np.abs(x32 - x64).max()),
+                    "slipped_area_fp64": float(area64), "slipped_area_rel_err": float(abs(area32 - area64) / abs(area64)),
+                    "mean_stress_max_abs_err": float(np.abs(s32 - s64).max()),
+                    "mean_stress_fp64": s64.tolist(),
+                    "step_roofline_frac_mixed": step_bytes(w.ns, w.P, False, True) * r32["steps_per_s"] / 8e12,
+                    "step_roofline_frac_fp64": step_bytes(w.ns, w.P, True, False) * r64["steps_per_s"] / 8e12})
+    else:
+        w = C.c4(steps=a.steps or 5000)
+        steps = w.steps
+        print(f"[run_config] c4 {w.shape} x {w.ns}, {steps} steps, setup {time.time() - t0:.1f}s", flush=True)
+        r32, x32, c32 = run_ramp(w, steps, MIXED, B_RECOMPUTE, a.every)
+        print(f"[run_config] mixed recompute {r32['steps_per_s']:.2f} steps/s", flush=True)
+        r64, x64, c64 = run_ramp(w, steps, FP64, B_RECOMPUTE, a.every)
+        print(f"[run_config] fp64 recompute {r64['steps_per_s']:.2f} steps/s", flush=True)
+        win = min(steps, 200)
+        rs, _, _ = run_ramp(C.c4(B_STORED, steps=steps), win, MIXED, B_STORED, win)
+        rr, _, _ = run_ramp(w, win, MIXED, B_RECOMPUTE, win)
+        print(f"[run_config] stored {rs['steps_per_s']:.2f} vs recompute {rr['steps_per_s']:.2f} steps/s", flush=True)
+        curve_err = max(abs(np.array(p["avesigma"]) - np.array(q["avesigma"])).max() for p, q in zip(c32, c64))
+        ceps_err = max(abs(np.array(p["aveps"]) - np.array(q["aveps"])).max() for p, q in zip(c32, c64))
+        res.update({"shape": w.shape, "ns": w.ns, "steps": steps, "every": a.every,
+                    "mixed_recompute": r32, "fp64_recompute": r64,
+                    "stored_vs_recompute_window": {"steps": win, "mixed_stored_steps_per_s": rs["steps_per_s"],
+                                                   "mixed_recompute_steps_per_s": rr["steps_per_s"],
+                                                   "stored_GB_s": step_bytes(3, w.P, False, True) * rs["steps_per_s"] / 1e9,
+                                                   "recompute_GB_s": step_bytes(3, w.P, False, False) * rr["steps_per_s"] / 1e9},
+                    "xi_max_abs_err": float(np.abs(x32 - x64).max()),
+                    "stress_strain_max_abs_err": {"avesigma": float(curve_err), "aveps": float(ceps_err)},
+                    "curve_fp64": c64, "curve_mixed": c32})
+    res["wall_s"] = time.time() - t0
+    txt = json.dumps(res)
+    if a.json:
+        with open(a.json, "w") as fh:
+            fh.write(txt)
+    summary = {k: v for k, v in res.items() if not k.startswith("curve")}
+    print(json.dumps(summary), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

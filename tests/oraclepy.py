@@ -51,6 +51,7 @@ def lib():
         L.orc_set_xi.argtypes = [vp, vp, vp]
         L.orc_get_xi.argtypes = [vp, vp]
         L.orc_set_tau.argtypes = [vp, vp, ci]
+        L.orc_set_tau_ramp.argtypes = [vp, vp, vp]
         L.orc_prologue.argtypes = [vp, ci]
         L.orc_gl_steps.argtypes = [vp, ci, vp, vp]
         L.orc_plasticevolv.argtypes = [ci, ctypes.c_long, vp, vp, vp, vp, vp, vp, vp, ci, cd, cd, vp]
@@ -170,6 +171,11 @@ class Oracle:
     def set_tau(self, tau, per_point=False):
         t = np.ascontiguousarray(tau, dtype=np.float64)
         lib().orc_set_tau(self.h, t.ctypes.data, 1 if per_point else 0)
+
+    def set_tau_ramp(self, tau0, rate):
+        t = np.ascontiguousarray(tau0, dtype=np.float64)
+        r = np.ascontiguousarray(rate, dtype=np.float64)
+        lib().orc_set_tau_ramp(self.h, t.ctypes.data, r.ctypes.data)
 
     def prologue(self, quirk=True):
         lib().orc_prologue(self.h, 1 if quirk else 0)

@@ -408,3 +408,61 @@ def test_kernel_plans_agree(plan, prec):
         assert p1[pa.STAGE_FFT_AXIS3] == 2
     if plan & pa.PLAN_FUSED_COLS:
         assert p1[pa.STAGE_FFT_AXIS1] == 0
+
+
+def test_tau_ramp_vs_oracle():
+    say("tau ramp vs oracle")
+    w = C.small_fcc((16, 16, 16), ns=3)
+    rate = np.array([2e-4, -1e-4, 3e-4])
+    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * 3, C.asf2_reference(3), [1.0] * 3)
+    o = O.Oracle(p)
+    o.set_xi(w.xi)
+    o.set_tau_ramp(w.tau, rate)
+    o.prologue(True)
+    go, gao = o.gl_steps(30)
+    with pa.Context(w.params()) as c:
+        c.set_tau_ramp(w.tau, rate)
+        c.upload_xi(w.xi)
+        c.begin_outer(True)
+        g1, ga1 = c.gl_steps(17)        # crosses a graph chunk boundary and a gl_steps call
+        g2, ga2 = c.gl_steps(13)
+        xg = c.download_xi()
+    assert np.abs(np.concatenate([ga1, ga2]) - gao).max() <= 1e-12
+    assert np.abs(xg - o.get_xi()).max() <= 1e-12
+
+
+def test_c2_mixed_vs_fp64_short():
+    """c2 shape (256^3 x 12, 64 loops), 200 steps: mixed with stored FP32 B
+    against FP64 with recomputed B."""
+    say("c2 256^3 x 12, 200 steps, mixed vs fp64")
+    w = C.c2()
+    x64, g64, ga64 = run_gpu(w, 200, precision=pa.FP64, bmode=pa.B_RECOMPUTE)
+    x32, g32, ga32 = run_gpu(w, 200, precision=pa.MIXED, bmode=pa.B_STORED)
+    assert np.abs(x32 - x64).max() <= 1e-4
+    a64, a32 = ga64[-1].sum(), ga32[-1].sum()
+    assert abs(a32 - a64) <= 1e-3 * abs(a64)
+
+
+def test_c4_ramp_mixed_vs_fp64_short():
+    """c4-shaped ramp (512^3 x 3, 8 loops r in [20,60]) over 100 steps: the
+    mixed and FP64 stress-strain samples agree."""
+    say("c4 512^3 x 3 ramp, 100 steps, mixed vs fp64")
+    w = C.c4(steps=100)
+    curves = {}
+    for prec in (pa.FP64, pa.MIXED):
+        with pa.Context(w.params(prec, pa.B_RECOMPUTE)) as c:
+            c.set_tau_ramp(w.tau, w.tau_rate)
+            c.upload_xi(w.xi)
+            c.begin_outer(True)
+            pts = []
+            for k in range(2):
+                _, ga = c.gl_steps(50)
+                sig = w.sigma_unit * 0.03 * (50 * (k + 1) - 1) / 99
+                ae, asg = c.strain_stress(w.s_tensor_avepst(sig))
+                c.begin_outer(False)
+                pts.append((ae, asg, ga[-1]))
+            curves[prec] = pts
+    for (e64, s64, g64), (e32, s32, g32) in zip(curves[pa.FP64], curves[pa.MIXED]):
+        assert np.abs(s32 - s64).max() <= 1e-6
+        assert np.abs(e32 - e64).max() <= 1e-6
+        assert np.abs(g32 - g64).max() <= 1e-6

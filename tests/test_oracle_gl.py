@@ -95,3 +95,22 @@ def test_gl_loop_matches_reference_functions(quirk):
         assert np.abs(g - gr).max() < 1e-8
     else:
         assert err > 1e-2
+
+
+def test_tau_ramp_uniform_closed_form():
+    """Uniform xi with a linear tau ramp: step s uses tau0 + rate*s."""
+    n = (8, 6, 4)
+    eps = ph.fcc_eps([0, 1, 2])
+    p = O.make_params(n, eps, ph.C11, ph.C12, ph.C44, [0.1] * 3, ph.asf2_reference(3), [1.0] * 3)
+    o = O.Oracle(p)
+    x0 = np.array([0.1, 0.5, 0.9])
+    o.set_xi(np.ones((3,) + n) * x0[:, None, None, None])
+    tau0, rate = np.array([0.0, 0.01, -0.02]), np.array([1e-3, -2e-4, 5e-4])
+    o.set_tau_ramp(tau0, rate)
+    o.prologue(True)
+    _, ga = o.gl_steps(15)
+    x = x0.copy()
+    asf = np.array(ph.asf2_reference(3))
+    for s in range(15):
+        x = x - 0.1 * (asf * ph.PI_REF * np.sin(2.0 * ph.PI_REF * x) - (tau0 + rate * s))
+        assert np.abs(ga[s] - x).max() < 1e-13
