@@ -67,7 +67,9 @@ enum {
   PFDD_PLAN_AUTO = 0,
   PFDD_PLAN_UNFUSED_ROWS = 1,  /* plain N3 passes + pointwise update instead of the fused row kernel */
   PFDD_PLAN_UNFUSED_COLS = 2,  /* plain N1 passes + pointwise contraction instead of the fused column kernel */
-  PFDD_PLAN_FUSED_COLS = 4     /* fused column kernel whenever its tile fits in LDS */
+  PFDD_PLAN_FUSED_COLS = 4,    /* fused column kernel whenever its tile fits in LDS */
+  PFDD_PLAN_NO_PLANE = 8,      /* no LDS-resident N2 x N3 plane kernel (AUTO uses it for N2 = N3 <= 96) */
+  PFDD_PLAN_PLANE = 16         /* the plane kernel also at 128^2 (mixed precision only) */
 };
 
 typedef struct pfdd_ctx pfdd_ctx;

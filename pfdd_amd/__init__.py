@@ -18,7 +18,7 @@ OK, E_HIP, E_ALLOC, E_SHAPE, E_RADIX, E_NAN, E_UNSUPPORTED, E_STATE = 0, -1, -2,
 FP64, MIXED = 0, 1
 B_STORED, B_RECOMPUTE = 0, 1
 FORWARD, BACKWARD = -1, 1
-PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS = 0, 1, 2, 4
+PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS, PLAN_NO_PLANE, PLAN_PLANE = 0, 1, 2, 4, 8, 16
 
 EXPORTED = [
     "pfdd_create", "pfdd_destroy", "pfdd_set_tau", "pfdd_upload_xi", "pfdd_download_xi",

@@ -59,6 +59,7 @@ struct pfdd_ctx {
   double* small = nullptr;   // reduction outputs (<= 32 doubles)
   double* avepst_d = nullptr;
   ElasticConsts* ec = nullptr;
+  double* freq = nullptr;    // frec tables of the three axes, n1 + n2 + n3 doubles
   void* E6 = nullptr;        // strain spectrum [6][P], allocated on first use
   double* efield = nullptr;  // [9][P] strain/stress fields, on demand
   double* sfield = nullptr;
@@ -68,6 +69,7 @@ struct pfdd_ctx {
   int row_tile = 0;          // rows per block of the fused row kernel (0: not fused)
   int row_nblk = 0;          // its blocks per field
   int c3_tile = 0;           // columns per block of the fused column kernel (0: not fused)
+  int plane = 0;             // 1: the LDS-resident plane kernel does the N2/N3 passes and the update
   size_t c3_lds = 0;         // its dynamic LDS bytes
   int has_xi = 0;
   int prologue_done = 0;
@@ -250,6 +252,76 @@ int k3_launch(pfdd_ctx* c, int quirk, bool attr_only = false) {
   }
 }
 
+// The plane kernel exists for square N2 = N3 planes that fit in LDS.
+bool plane_len(int n, bool fp64) {
+  if (fp64) return n == 32 || n == 48 || n == 64 || n == 96;
+  return n == 32 || n == 48 || n == 64 || n == 96 || n == 128;
+}
+
+template <typename T, int N>
+int plane_launch_n(pfdd_ctx* c, bool attr_only) {
+  constexpr size_t lds = plane_lds_bytes<T, N, N>();
+  const cx<T>* tw2 = tw_of<T>(c->tw[1]);
+  const cx<T>* tw3 = tw_of<T>(c->tw[2]);
+  const unsigned nb = (unsigned)(c->S * c->p.n1);
+  cx<T>* W = (cx<T>*)c->W;
+#define PFDD_PL(MM, TT)                                                                                          \
+  if (attr_only) {                                                                                               \
+    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_step<T, N, N, MM, TT>),                     \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                           \
+  } else {                                                                                                       \
+    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT>), dim3(nb), dim3(PLANE_THREADS), lds, c->stream, W, c->xi,  \
+                       c->mask, c->taupp, c->sa, tw2, tw3, c->partials, c->step_ctr);                           \
+  }
+  if (attr_only) {
+    PFDD_PL(true, true) PFDD_PL(true, false) PFDD_PL(false, true) PFDD_PL(false, false)
+  } else {
+    const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+    if (m && tp) { PFDD_PL(true, true) }
+    else if (m) { PFDD_PL(true, false) }
+    else if (tp) { PFDD_PL(false, true) }
+    else { PFDD_PL(false, false) }
+  }
+#undef PFDD_PL
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+template <typename T>
+int plane_launch(pfdd_ctx* c, bool finish = true, bool attr_only = false) {
+  int rc;
+  switch (c->p.n3) {
+#define PFDD_L(NN) case NN: rc = plane_launch_n<T, NN>(c, attr_only); break;
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96)
+#undef PFDD_L
+    case 128:
+      if (sizeof(T) == 4) {
+        rc = plane_launch_n<float, 128>(c, attr_only);
+        break;
+      }
+      return PFDD_E_RADIX;
+    default:
+      return PFDD_E_RADIX;
+  }
+  if (rc || attr_only) return rc;
+  if (finish)
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->p.n1, c->hist,
+                       c->step_ctr);
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+// The N1 part of the elastic solve: forward N1, contraction, inverse N1
+// (fused into one kernel when it fits).
+template <typename T>
+int n1_part(pfdd_ctx* c, int quirk) {
+  int rc;
+  cx<T>* W = (cx<T>*)c->W;
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
+  if (c->c3_tile > 0) return k3_launch<T>(c, quirk);
+  if ((rc = launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_FORWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
+  if ((rc = contract<T>(c, quirk))) return rc;
+  return launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream);
+}
+
 // Passes between the forward N3 pass and the inverse N3 pass: forward N2,
 // then (forward N1, contraction, inverse N1) -- fused when it fits -- then
 // inverse N2.
@@ -259,13 +331,7 @@ int middle(pfdd_ctx* c, int quirk) {
   cx<T>* W = (cx<T>*)c->W;
   const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
   if ((rc = launch_axis<T>(W, (long)S * n1, n2, n3, PFDD_FORWARD, tw_of<T>(c->tw[1]), c->stream))) return rc;
-  if (c->c3_tile > 0) {
-    if ((rc = k3_launch<T>(c, quirk))) return rc;
-  } else {
-    if ((rc = launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_FORWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
-    if ((rc = contract<T>(c, quirk))) return rc;
-    if ((rc = launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
-  }
+  if ((rc = n1_part<T>(c, quirk))) return rc;
   return launch_axis<T>(W, (long)S * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream);
 }
 
@@ -281,6 +347,10 @@ int axis3(pfdd_ctx* c, int sign) {
 template <typename T>
 int one_step(pfdd_ctx* c) {
   int rc;
+  if (c->plane) {
+    if ((rc = plane_launch<T>(c))) return rc;
+    return n1_part<T>(c, 0);
+  }
   if (c->row_tile > 0) {
     if ((rc = k1_launch<T>(c))) return rc;
   } else {
@@ -298,6 +368,12 @@ int prologue(pfdd_ctx* c, int quirk) {
   const long n = (long)c->S * c->P;
   hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, (cx<T>*)c->W, n);
   if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
+  if (c->plane) {  // the next step's plane kernel finishes the N2 and N3 inverse passes
+    if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->S * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD, tw_of<T>(c->tw[1]),
+                             c->stream)))
+      return rc;
+    return n1_part<T>(c, quirk);
+  }
   if ((rc = middle<T>(c, quirk))) return rc;
   if (c->row_tile == 0) return axis3<T>(c, PFDD_BACKWARD);
   return PFDD_OK;
@@ -444,7 +520,7 @@ void pfdd_destroy(pfdd_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void* bufs[] = {c->xi, c->W, c->Bsym, c->mask, c->taupp, c->partials, c->hist, c->step_ctr,
-                  c->small, c->avepst_d, c->ec, c->E6, c->efield, c->sfield};
+                  c->small, c->avepst_d, c->ec, c->E6, c->efield, c->sfield, c->freq};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& t : c->tw) {
@@ -494,7 +570,9 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     const long strain_nblk = (P + 255) / 256;
     // fused row kernel: rows per block = the largest power of two <= its
     // compile-time tile that divides the N1*N2 rows of a field
-    if (p->plan & ~(PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS)) FAIL(PFDD_E_SHAPE);
+    if (p->plan & ~(PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS | PFDD_PLAN_NO_PLANE |
+                    PFDD_PLAN_PLANE))
+      FAIL(PFDD_E_SHAPE);
     if (fused_len(p->n3) && !(p->plan & PFDD_PLAN_UNFUSED_ROWS)) {
       const long rows = (long)p->n1 * p->n2;
       const int tmax = row_tile_max(p->n3, c->fp64);
@@ -515,8 +593,13 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
         c->c3_lds = t * per_col;
       }
     }
+    // the plane kernel holds one block per CU; at 128^2 (FP32) it measured no faster than the row kernel +
+    // two N2 passes (0.36 vs 0.35 ms at 128^3 x 12), so AUTO takes it up to 96^2 and PFDD_PLAN_PLANE forces it
+    if (!(p->plan & PFDD_PLAN_NO_PLANE) && p->n2 == p->n3 && plane_len(p->n3, c->fp64) &&
+        (p->n3 <= 96 || (p->plan & PFDD_PLAN_PLANE)))
+      c->plane = 1;
     c->partials_len = (size_t)std::max<long>(std::max<long>((long)S * c->upd_nblk, (long)S * c->row_nblk),
-                                             18 * strain_nblk);
+                                             std::max<long>(18 * strain_nblk, (long)S * p->n1));
     if (!(c->xi = (double2*)dev_alloc<char>(sizeof(double2) * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->W = dev_alloc<char>(csz * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -561,6 +644,26 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
         h.SE[a][b] = se;
       }
     h.d[0] = p->d1; h.d[1] = p->d2; h.d[2] = p->d3;
+    // frec (G.c:2698-2741) per axis, evaluated with the same arithmetic as the reference
+    std::vector<double> fr((size_t)p->n1 + p->n2 + p->n3);
+    {
+      size_t o = 0;
+      const int nn[3] = {p->n1, p->n2, p->n3};
+      for (int ax = 0; ax < 3; ax++)
+        for (int i = 0; i < nn[ax]; i++, o++) {
+          const double dd = h.d[ax];
+          fr[o] = i == 0 ? 0.0 : i < nn[ax] / 2 ? (double)i / (double)nn[ax] / dd
+                                                 : ((double)i - (double)nn[ax]) / (double)nn[ax] / dd;
+        }
+    }
+    if (!(c->freq = (double*)dev_alloc<char>(sizeof(double) * fr.size(), c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (hipMemcpyAsync(c->freq, fr.data(), sizeof(double) * fr.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      FAIL(PFDD_E_HIP);
+    h.freq[0] = c->freq;
+    h.freq[1] = c->freq + p->n1;
+    h.freq[2] = c->freq + p->n1 + p->n2;
+    h.iso = h.mup == 0.0;
+    h.g_iso = (h.mu + h.ll) / (h.mu * h.mu * (1.0 + (h.mu + h.ll) / h.mu));
     h.n[0] = p->n1; h.n[1] = p->n2; h.n[2] = p->n3;
     h.ns = S;
     if (hipMemcpyAsync(c->ec, &h, sizeof(h), hipMemcpyHostToDevice, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
@@ -591,6 +694,10 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     }
     if (c->c3_tile > 0) {
       rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true);
+      if (rc) goto fail;
+    }
+    if (c->plane) {
+      rc = c->fp64 ? plane_launch<double>(c, false, true) : plane_launch<float>(c, false, true);
       if (rc) goto fail;
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
@@ -790,11 +897,11 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
     const int sign = (r & 1) ? PFDD_BACKWARD : PFDD_FORWARD;
     switch (stage) {
       case PFDD_STAGE_UPDATE:
-        rc = c->row_tile > 0 ? k1_launch<T>(c, false) : update<T>(c, false);
+        rc = c->plane ? plane_launch<T>(c, false) : c->row_tile > 0 ? k1_launch<T>(c, false) : update<T>(c, false);
         break;
       case PFDD_STAGE_FINISH:
         hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S,
-                           c->row_tile > 0 ? c->row_nblk : c->upd_nblk, c->hist, (int*)nullptr);
+                           c->plane ? c->p.n1 : c->row_tile > 0 ? c->row_nblk : c->upd_nblk, c->hist, (int*)nullptr);
         rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
         break;
       case PFDD_STAGE_FFT_AXIS3:
@@ -828,10 +935,10 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
 
 extern "C" int pfdd_step_plan(const pfdd_ctx* c, int* launches) {
   if (!c || !launches) return PFDD_E_SHAPE;
-  const bool r = c->row_tile > 0, k3 = c->c3_tile > 0;
+  const bool r = c->row_tile > 0 || c->plane, k3 = c->c3_tile > 0;
   launches[PFDD_STAGE_UPDATE] = 1;
   launches[PFDD_STAGE_FFT_AXIS3] = r ? 0 : 2;
-  launches[PFDD_STAGE_FFT_AXIS2] = 2;
+  launches[PFDD_STAGE_FFT_AXIS2] = c->plane ? 0 : 2;
   launches[PFDD_STAGE_FFT_AXIS1] = k3 ? 0 : 2;
   launches[PFDD_STAGE_CONTRACT] = 1;
   launches[PFDD_STAGE_STEP] = 0;

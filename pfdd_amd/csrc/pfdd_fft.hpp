@@ -119,14 +119,51 @@ template <typename T> struct Bfly<T, 8> {
   }
 };
 
+// ---------------------------------------------------------------- pencil access
+// A transform reads its input and writes its output through accessors, so
+// the first Stockham pass can load straight from HBM into registers and the
+// last pass can store straight back; only the passes in between go through
+// LDS.  Element i of pencil p:
+template <typename T> struct LdsPencils {     // sm[p * ld + i]
+  cx<T>* sm;
+  int ld;
+  static constexpr bool in_lds = true;
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return sm[p * ld + i]; }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { sm[p * ld + i] = v; }
+};
+template <typename T> struct RowPencils {      // contiguous rows: base[p * n + i]
+  cx<T>* base;
+  int n;
+  static constexpr bool in_lds = false;
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return base[(long)p * n + i]; }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { base[(long)p * n + i] = v; }
+};
+template <typename T> struct ColPencils {      // strided columns: base[i * inner + p]
+  cx<T>* base;
+  long inner;
+  static constexpr bool in_lds = false;
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return base[(long)i * inner + p]; }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { base[(long)i * inner + p] = v; }
+};
+
+template <typename T> struct LdsColPencils {  // columns of a row-major LDS plane: sm[i * ld + p]
+  cx<T>* sm;
+  int ld;
+  static constexpr bool in_lds = true;
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return sm[i * ld + p]; }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { sm[i * ld + p] = v; }
+};
+
 // ---------------------------------------------------------------- passes
-// One Stockham pass on G pencils per thread.  Pencil g of this thread starts
-// at sm + (p0 + g * pstride) * LD and is processed when p0 + g * pstride < npen;
-// q in [0, TPP) is the thread's lane within the pencil.  tw[m] = exp(+2 pi i m / N);
-// the sign s flips its imaginary part.  Every thread of the block must call it.
-template <typename T, int N, int R, int NSB, int TPP, int G, int LD>
-__device__ __forceinline__ void stockham_pass(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
-                                              const cx<T>* __restrict__ tw) {
+// One Stockham pass on G pencils per thread, reading through `in` and
+// writing through `out`.  Pencil g of this thread is p0 + g * pstride and is
+// processed when < npen; q in [0, TPP) is the thread's lane within the
+// pencil.  tw[m] = exp(+2 pi i m / N); the sign s flips its imaginary part.
+// Every thread of the block calls it: there is a barrier between the read
+// and the write phase, and one after the write phase when out is in LDS.
+template <typename T, int N, int R, int NSB, int TPP, int G, typename In, typename Out>
+__device__ __forceinline__ void stockham_pass(const In& in, const Out& out, int p0, int pstride, int npen, int q,
+                                              T s, const cx<T>* __restrict__ tw) {
   constexpr int M = N / R;
   constexpr int J = (M + TPP - 1) / TPP;
   constexpr int STEP = N / (NSB * R);
@@ -135,13 +172,12 @@ __device__ __forceinline__ void stockham_pass(cx<T>* sm, int p0, int pstride, in
   for (int g = 0; g < G; ++g) {
     const int p = p0 + g * pstride;
     if (p >= npen) continue;
-    cx<T>* pen = sm + p * LD;
 #pragma unroll
     for (int jj = 0; jj < J; ++jj) {
       const int j = q + jj * TPP;
       if ((M % TPP == 0) || j < M) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[g][jj][r] = pen[j + r * M];
+        for (int r = 0; r < R; ++r) v[g][jj][r] = in.load(p, j + r * M);
         if (NSB > 1) {
           const int k = j % NSB;
 #pragma unroll
@@ -155,48 +191,67 @@ __device__ __forceinline__ void stockham_pass(cx<T>* sm, int p0, int pstride, in
       }
     }
   }
-  __syncthreads();
+  if (In::in_lds || Out::in_lds) __syncthreads();
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int p = p0 + g * pstride;
     if (p >= npen) continue;
-    cx<T>* pen = sm + p * LD;
 #pragma unroll
     for (int jj = 0; jj < J; ++jj) {
       const int j = q + jj * TPP;
       if ((M % TPP == 0) || j < M) {
         const int d = (j / NSB) * NSB * R + (j % NSB);
 #pragma unroll
-        for (int r = 0; r < R; ++r) pen[d + r * NSB] = v[g][jj][r];
+        for (int r = 0; r < R; ++r) out.store(p, d + r * NSB, v[g][jj][r]);
       }
     }
   }
-  __syncthreads();
+  if (Out::in_lds) __syncthreads();
 }
 
-template <typename T, int N, int G, int LD, int K>
-__device__ __forceinline__ void fft_lds_from(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
-                                             const cx<T>* __restrict__ tw) {
+template <typename T, int N, int G, int K, typename In, typename Mid, typename Out>
+__device__ __forceinline__ void fft_from(const In& in, const Mid& lds, const Out& out, int p0, int pstride,
+                                         int npen, int q, T s, const cx<T>* __restrict__ tw) {
   constexpr Plan pl = make_plan(N);
   if constexpr (K < pl.np) {
-    stockham_pass<T, N, pl.r[K], ns_before(N, K), pl.tpp, G, LD>(sm, p0, pstride, npen, q, s, tw);
-    fft_lds_from<T, N, G, LD, K + 1>(sm, p0, pstride, npen, q, s, tw);
+    constexpr int R = pl.r[K], NSB = ns_before(N, K), TPP = pl.tpp;
+    if constexpr (pl.np == 1) {
+      stockham_pass<T, N, R, NSB, TPP, G>(in, out, p0, pstride, npen, q, s, tw);
+    } else if constexpr (K == 0) {
+      stockham_pass<T, N, R, NSB, TPP, G>(in, lds, p0, pstride, npen, q, s, tw);
+    } else if constexpr (K == pl.np - 1) {
+      stockham_pass<T, N, R, NSB, TPP, G>(lds, out, p0, pstride, npen, q, s, tw);
+    } else {
+      stockham_pass<T, N, R, NSB, TPP, G>(lds, lds, p0, pstride, npen, q, s, tw);
+    }
+    fft_from<T, N, G, K + 1>(in, lds, out, p0, pstride, npen, q, s, tw);
   }
 }
 
-// N-point transforms of up to G pencils per thread (pencil p at sm + p*LD).
-// The caller synchronises after filling sm; the function ends synchronised.
+// N-point transforms of up to G pencils per thread: the first pass reads
+// through `in`, the last writes through `out`, the passes in between use
+// `lds` (which needs npen pencils of pitch >= N).  When the input is already
+// in LDS the caller synchronises first; the function ends synchronised when
+// the output is in LDS.
+template <typename T, int N, int G, typename In, typename Mid, typename Out>
+__device__ __forceinline__ void fft_pencils(const In& in, const Mid& lds, const Out& out, int p0,
+                                            int pstride, int npen, int q, T s, const cx<T>* __restrict__ tw) {
+  fft_from<T, N, G, 0>(in, lds, out, p0, pstride, npen, q, s, tw);
+}
+
+// All-LDS transforms of up to G pencils per thread (pencil p at sm + p*LD).
 template <typename T, int N, int G, int LD>
 __device__ __forceinline__ void fft_lds_multi(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
                                               const cx<T>* __restrict__ tw) {
-  fft_lds_from<T, N, G, LD, 0>(sm, p0, pstride, npen, q, s, tw);
+  const LdsPencils<T> l{sm, LD};
+  fft_pencils<T, N, G>(l, l, l, p0, pstride, npen, q, s, tw);
 }
 
-// One pencil per thread group: pencil p = threadIdx.x / TPP at sm + p*LD.
+// One pencil per TPP-thread group: pencil p = threadIdx.x / TPP at sm + p*LD.
 template <typename T, int N, int LD>
 __device__ __forceinline__ void fft_lds(cx<T>* sm, int npen, T s, const cx<T>* __restrict__ tw) {
   constexpr int TPP = make_plan(N).tpp;
-  fft_lds_from<T, N, 1, LD, 0>(sm, (int)threadIdx.x / TPP, 1 << 30, npen, (int)threadIdx.x % TPP, s, tw);
+  fft_lds_multi<T, N, 1, LD>(sm, (int)threadIdx.x / TPP, 1 << 30, npen, (int)threadIdx.x % TPP, s, tw);
 }
 
 // LDS row pitch (in complex elements) for a pencil of N points.

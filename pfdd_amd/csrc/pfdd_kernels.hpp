@@ -28,6 +28,9 @@ struct ElasticConsts {
   double C[81];                    // C_ijkl
   double eps[MAXS][9];             // e_a
   double d[3];                     // grid spacing
+  const double* freq[3];           // per-axis frec tables (device), freq[a][i] = frec1(i, n[a], d[a])
+  double g_iso;                    // (mu+ll) / (mu^2 kk) for mup == 0
+  int iso;                         // mup == 0: isotropic Green's function
   int n[3];
   int ns;
 };
@@ -44,6 +47,16 @@ __device__ __forceinline__ double frec1(int i, int n, double d) {
 __device__ __forceinline__ bool green6(const double f[3], const ElasticConsts* __restrict__ ec, double g[6]) {
   const double fk2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
   if (!(fk2 > 0.0)) return false;
+  if (ec->iso) {  // mup = 0: kk = 1 + (mu+ll)/mu, G = (delta/mu - f f (mu+ll)/(mu^2 kk |f|^2)) / |f|^2
+    const double inv = 1 / fk2, c = ec->g_iso * inv, im = ec->inv_mu;
+    g[0] = inv * (im - f[0] * f[0] * c);
+    g[1] = inv * (-f[0] * f[1] * c);
+    g[2] = inv * (-f[0] * f[2] * c);
+    g[3] = inv * (im - f[1] * f[1] * c);
+    g[4] = inv * (-f[1] * f[2] * c);
+    g[5] = inv * (im - f[2] * f[2] * c);
+    return true;
+  }
   const double mu = ec->mu, ll = ec->ll, mup = ec->mup;
   const double z0 = f[0] * f[0] / fk2, z1 = f[1] * f[1] / fk2, z2 = f[2] * f[2] / fk2;
   const double m0 = mu + mup * z0, m1 = mu + mup * z1, m2 = mu + mup * z2;
@@ -71,9 +84,9 @@ __device__ __forceinline__ void kvec(long pt, const ElasticConsts* __restrict__ 
   const long r = pt / n3;
   const int j = (int)(r % n2);
   const int i = (int)(r / n2);
-  f[0] = frec1(i, ec->n[0], ec->d[0]);
-  f[1] = frec1(j, n2, ec->d[1]);
-  f[2] = frec1(k, n3, ec->d[2]);
+  f[0] = ec->freq[0][i];
+  f[1] = ec->freq[1][j];
+  f[2] = ec->freq[2][k];
 }
 
 __device__ __forceinline__ int tri_index(int a, int b, int S) {
@@ -121,42 +134,35 @@ template <typename T, int N> constexpr int tile_cols() {
   return t;
 }
 
-// Transform along the contiguous axis: rows of N consecutive elements.
+// Transform along the contiguous axis: rows of N consecutive elements.  The
+// first pass reads the rows from HBM, the last writes them back.
 template <typename T, int N>
 __global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
 k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict__ tw) {
-  constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), NT = TILE * TPP, LD = pitch<N>();
+  constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), LD = pitch<N>();
   __shared__ cx<T> sm[TILE * LD];
   const long row0 = (long)blockIdx.x * TILE;
   const int nr = (int)((nrows - row0) < TILE ? (nrows - row0) : TILE);
-  cx<T>* base = data + row0 * N;
-  for (int e = threadIdx.x; e < nr * N; e += NT) sm[(e / N) * LD + (e % N)] = base[e];
-  __syncthreads();
-  fft_lds<T, N, LD>(sm, nr, sign, tw);
-  for (int e = threadIdx.x; e < nr * N; e += NT) base[e] = sm[(e / N) * LD + (e % N)];
+  const RowPencils<T> g{data + row0 * N, N};
+  const LdsPencils<T> l{sm, LD};
+  fft_pencils<T, N, 1>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, tw);
 }
 
 // Transform along a strided axis of an [outer][N][inner] array; each block
 // takes `tile` consecutive inner indices (tile divides inner, tile <= TILE).
+// Threads run pencil-fastest, so a wave's loads of one row of the tile are
+// contiguous.
 template <typename T, int N>
 __global__ void __launch_bounds__((tile_cols<T, N>() * make_plan(N).tpp))
 k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* __restrict__ tw) {
-  constexpr int TPP = make_plan(N).tpp, TILE = tile_cols<T, N>(), NT = TILE * TPP, LD = pitch<N>();
+  constexpr int TILE = tile_cols<T, N>(), LD = pitch<N>();
   __shared__ cx<T> sm[TILE * LD];
   const long tiles_per_outer = inner / tile;
   const long o = blockIdx.x / tiles_per_outer;
   const long i0 = (blockIdx.x % tiles_per_outer) * tile;
-  cx<T>* base = data + o * N * inner + i0;
-  for (int e = threadIdx.x; e < N * tile; e += NT) {
-    const int n = e / tile, p = e % tile;
-    sm[p * LD + n] = base[(long)n * inner + p];
-  }
-  __syncthreads();
-  fft_lds<T, N, LD>(sm, tile, sign, tw);
-  for (int e = threadIdx.x; e < N * tile; e += NT) {
-    const int n = e / tile, p2 = e % tile;
-    base[(long)n * inner + p2] = sm[p2 * LD + n];
-  }
+  const ColPencils<T> g{data + o * N * inner + i0, inner};
+  const LdsPencils<T> l{sm, LD};
+  fft_pencils<T, N, 1>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, tw);
 }
 
 // ---------------------------------------------------------------- GL kernels
@@ -416,9 +422,10 @@ k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __res
   const long off = (long)f * P + pt0;
   cx<T>* wb = W + off;
   double2* xb = xi + off;
-  for (int e = threadIdx.x; e < tile * N; e += NT) sm[(e / N) * LD + (e % N)] = wb[e];
-  __syncthreads();
-  fft_lds<T, N, LD>(sm, tile, (T)1, tw);
+  const RowPencils<T> g{wb, N};
+  const LdsPencils<T> l{sm, LD};
+  const int pen = (int)threadIdx.x / TPP, q = (int)threadIdx.x % TPP;
+  fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, tw);
   const double nsize = (double)P;
   const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
   const double rate = a.tau_rate[f] * (double)step_ctr[1];
@@ -438,10 +445,73 @@ k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __res
     acc += x.x / nsize;
   }
   __syncthreads();
-  fft_lds<T, N, LD>(sm, tile, (T)-1, tw);
-  for (int e = threadIdx.x; e < tile * N; e += NT) wb[e] = sm[(e / N) * LD + (e % N)];
+  fft_pencils<T, N, 1>(l, l, g, pen, 1 << 30, tile, q, (T)-1, tw);
   const double sum = block_sum(acc, red);
   if (threadIdx.x == 0) partials[(long)f * (rows_per_field / tile) + (pt0 / N) / tile] = sum;
+}
+
+// Plane kernel: one block owns the (i2, i3) plane of one field at one i1 and
+// keeps it in LDS (row-major, pitch N3+1).  It finishes the inverse
+// transform along N2 and N3 (the first N2 pass reads HBM), applies
+// plasticevolv (G.c:3470-3501) and the copy to the FFT array, then runs the
+// forward N3 and N2 passes (the last one writes HBM): with the N1 passes and
+// the contraction this is the whole GL step in two HBM round trips of W.
+// Block sum of Re xi / P -> partials[f][i1].
+constexpr int PLANE_THREADS = 1024;
+template <typename T, int N2, int N3>
+constexpr size_t plane_lds_bytes() { return (size_t)N2 * (N3 + 1) * sizeof(cx<T>); }
+
+template <typename T, int N2, int N3, bool MASK, bool TAUPP>
+__global__ void __launch_bounds__(PLANE_THREADS)
+k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
+             const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
+             const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
+  constexpr int LD = N3 + 1, PLANE = N2 * N3;
+  constexpr int TPP2 = make_plan(N2).tpp, TPP3 = make_plan(N3).tpp;
+  constexpr int PPI2 = PLANE_THREADS / TPP2, PPI3 = PLANE_THREADS / TPP3;
+  constexpr int G2 = (N3 + PPI2 - 1) / PPI2, G3 = (N2 + PPI3 - 1) / PPI3;  // N3 column pencils, N2 row pencils
+  extern __shared__ unsigned char lds_raw[];
+  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  __shared__ double red[PLANE_THREADS / 64];
+  const long P = a.P;
+  const int n1 = (int)(P / PLANE);
+  const int f = blockIdx.x / n1, i1 = blockIdx.x - f * n1;
+  const long off = (long)f * P + (long)i1 * PLANE;
+  const ColPencils<T> gcol{W + off, N3};
+  const LdsColPencils<T> lcol{sm, LD};
+  const LdsPencils<T> lrow{sm, LD};
+  const int t = threadIdx.x;
+  // threads past PPI*TPP (when TPP does not divide the block) sit out
+  const int c_p0 = t < PPI2 * TPP2 ? t % PPI2 : (1 << 30), c_q = t / PPI2;
+  const int r_p0 = t < PPI3 * TPP3 ? t / TPP3 : (1 << 30), r_q = t % TPP3;
+  // inverse along N2 (columns: pencil i3, pencil-fastest threads), HBM -> LDS
+  fft_pencils<T, N2, G2>(gcol, lcol, lcol, c_p0, PPI2, N3, c_q, (T)1, tw2);
+  // inverse along N3 (rows)
+  fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)1, tw3);
+  const double nsize = (double)P;
+  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
+  const double rate = a.tau_rate[f] * (double)step_ctr[1];
+  const double tauf = a.tau[f] + rate;
+  double2* xb = xi + off;
+  double acc = 0.0;
+  for (int e = t; e < PLANE; e += PLANE_THREADS) {
+    const int li = (e / N3) * LD + (e % N3);
+    const cx<T> d = sm[li];
+    double2 x = xb[e];
+    if (!MASK || mask[off + e] == 0) {
+      const double tt = TAUPP ? taupp[((long)i1 * PLANE + e) * a.ns + f] + rate : tauf;
+      x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - tt / dsl);
+      x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
+      xb[e] = x;
+    }
+    sm[li] = {(T)x.x, (T)x.y};
+    acc += x.x / nsize;
+  }
+  __syncthreads();
+  fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)-1, tw3);
+  fft_pencils<T, N2, G2>(lcol, lcol, gcol, c_p0, PPI2, N3, c_q, (T)-1, tw2);
+  const double sum = block_sum(acc, red);
+  if (t == 0) partials[(long)f * n1 + i1] = sum;
 }
 
 // K3: one block owns `tile` consecutive (i2,i3) columns of every field.

@@ -384,8 +384,10 @@ def test_fft_host_seam_matches_reference_fourn():
     L.pfdd_fft_host_release()
 
 
-@pytest.mark.parametrize("plan", [pa.PLAN_UNFUSED_ROWS, pa.PLAN_UNFUSED_COLS, pa.PLAN_FUSED_COLS,
-                                  pa.PLAN_UNFUSED_ROWS | pa.PLAN_FUSED_COLS])
+@pytest.mark.parametrize("plan", [pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_ROWS, pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_COLS,
+                                  pa.PLAN_NO_PLANE | pa.PLAN_FUSED_COLS,
+                                  pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_ROWS | pa.PLAN_FUSED_COLS,
+                                  pa.PLAN_FUSED_COLS, pa.PLAN_UNFUSED_COLS])
 @pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
 def test_kernel_plans_agree(plan, prec):
     """Every kernel plan computes the same GL step (fused or unfused passes)."""
@@ -404,8 +406,10 @@ def test_kernel_plans_agree(plan, prec):
     x1, p1 = run(plan)
     tol = 1e-13 if prec == pa.FP64 else 2e-6
     assert np.abs(x1 - x0).max() <= tol
-    if plan & pa.PLAN_UNFUSED_ROWS:
+    if plan & pa.PLAN_UNFUSED_ROWS and plan & pa.PLAN_NO_PLANE:
         assert p1[pa.STAGE_FFT_AXIS3] == 2
+    if not plan & pa.PLAN_NO_PLANE:
+        assert p1[pa.STAGE_FFT_AXIS2] == 0
     if plan & pa.PLAN_FUSED_COLS:
         assert p1[pa.STAGE_FFT_AXIS1] == 0
 
@@ -466,3 +470,54 @@ def test_c4_ramp_mixed_vs_fp64_short():
         assert np.abs(s32 - s64).max() <= 1e-6
         assert np.abs(e32 - e64).max() <= 1e-6
         assert np.abs(g32 - g64).max() <= 1e-6
+
+
+@pytest.mark.parametrize("shape,prec", [((32, 32, 32), pa.FP64), ((48, 48, 48), pa.FP64), ((8, 96, 96), pa.FP64),
+                                        ((16, 64, 64), pa.MIXED), ((12, 128, 128), pa.MIXED)])
+def test_plane_kernel_matches_oracle_and_unfused(shape, prec):
+    """The LDS-resident plane kernel (N2 = N3) against the oracle (FP64) and
+    against the plan without it."""
+    say(f"plane kernel {shape} prec {prec}")
+    w = C.small_fcc(shape, ns=5)
+
+    def run(plan):
+        with pa.Context(w.params(prec, pa.B_RECOMPUTE, plan)) as c:
+            c.set_tau(w.tau)
+            c.upload_xi(w.xi)
+            c.begin_outer(True)
+            _, ga = c.gl_steps(12)
+            return c.download_xi(), ga, c.step_plan()
+
+    xp, gp, plan_p = run(pa.PLAN_PLANE)
+    xn, gn, plan_n = run(pa.PLAN_NO_PLANE)
+    assert plan_p[pa.STAGE_FFT_AXIS2] == 0 and plan_n[pa.STAGE_FFT_AXIS2] == 2
+    tol = 1e-13 if prec == pa.FP64 else 2e-6
+    assert np.abs(xp - xn).max() <= tol
+    if prec == pa.FP64:
+        xo, go, gao = run_oracle(w, 12)
+        assert np.abs(xp - xo).max() <= 1e-12
+        assert np.abs(gp - gao).max() <= 1e-12
+
+
+@pytest.mark.parametrize("bmode", [pa.B_RECOMPUTE, pa.B_STORED])
+def test_cubic_material_vs_oracle(bmode):
+    """Anisotropic (cubic, mu' != 0) elasticity: the general Green's function
+    of Bmatrix (G.c:386) on the GPU against the oracle."""
+    say(f"cubic material bmode={bmode}")
+    w = C.small_fcc((16, 24, 32), ns=4)
+    Cc = (1.684, 1.214, 0.754)   # Cu-like ratios, C11 != C12 + 2 C44
+    p = pa.make_params(w.shape, w.eps, *Cc, [0.1] * 4, C.asf2_reference(4), [1.0] * 4, bmode=bmode)
+    with pa.Context(p) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi)
+        c.begin_outer(True)
+        _, gag = c.gl_steps(15)
+        xg = c.download_xi()
+    po = O.make_params(w.shape, w.eps, *Cc, [0.1] * 4, C.asf2_reference(4), [1.0] * 4)
+    o = O.Oracle(po)
+    o.set_xi(w.xi)
+    o.set_tau(w.tau)
+    o.prologue(True)
+    _, gao = o.gl_steps(15)
+    assert np.abs(xg - o.get_xi()).max() <= 1e-12
+    assert np.abs(gag - gao).max() <= 1e-12
