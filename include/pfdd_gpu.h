@@ -58,19 +58,27 @@ typedef struct {
   double uq2;                  /* 1.0 (G.c:4195) */
   double pi_const;             /* 3.141592654 (G.c:95) */
   double eps[PFDD_MAX_NS][3][3]; /* eigenstrains e_a (seteps G.c:1912-1980) */
-  int plan;                    /* kernel plan: PFDD_PLAN_AUTO (0) or PFDD_PLAN_* flags */
+  int plan;                    /* kernel plan: PFDD_PLAN_AUTO (0) or PFDD_PLAN_* flags and tiles */
 } pfdd_params;
 
-/* Kernel plan flags (no reference counterpart; results are the same to
- * rounding for every plan).  AUTO picks the measured-fastest plan. */
+/* Kernel plan word (no reference counterpart; results are the same to
+ * rounding for every plan).  AUTO picks the plan measured fastest at
+ * 128^3 x 12; pfdd_autotune measures the candidates on the actual problem.
+ * Bits 0-4 are flags; bits 8-11 and 12-15 optionally fix the tile of the
+ * fused row / column kernel as log2(tile) + 1 (0 = the largest that fits);
+ * a tile that does not fit is rejected with PFDD_E_SHAPE. */
 enum {
   PFDD_PLAN_AUTO = 0,
   PFDD_PLAN_UNFUSED_ROWS = 1,  /* plain N3 passes + pointwise update instead of the fused row kernel */
   PFDD_PLAN_UNFUSED_COLS = 2,  /* plain N1 passes + pointwise contraction instead of the fused column kernel */
   PFDD_PLAN_FUSED_COLS = 4,    /* fused column kernel whenever its tile fits in LDS */
   PFDD_PLAN_NO_PLANE = 8,      /* no LDS-resident N2 x N3 plane kernel (AUTO uses it for N2 = N3 <= 96) */
-  PFDD_PLAN_PLANE = 16         /* the plane kernel also at 128^2 (mixed precision only) */
+  PFDD_PLAN_PLANE = 16,        /* the plane kernel also at 128^2 (mixed precision only) */
+  PFDD_PLAN_ROW_TILE_SHIFT = 8,  /* rows along N3 per block of the fused row kernel */
+  PFDD_PLAN_COL_TILE_SHIFT = 12  /* (i2,i3) columns per block of the fused column kernel */
 };
+#define PFDD_PLAN_ROW_TILE_LOG2(l) (((l) + 1) << PFDD_PLAN_ROW_TILE_SHIFT)
+#define PFDD_PLAN_COL_TILE_LOG2(l) (((l) + 1) << PFDD_PLAN_COL_TILE_SHIFT)
 
 typedef struct pfdd_ctx pfdd_ctx;
 
@@ -131,6 +139,18 @@ int pfdd_strain_stress(pfdd_ctx *c, const double *avepst_uniform, double *aveps,
  * dir = PFDD_FORWARD (-1) or PFDD_BACKWARD (+1).  Asynchronous. */
 int pfdd_fft(pfdd_ctx *c, void *dev_buf, int batch, int dir, int prec);
 
+/* Kernel plan (no reference counterpart): apply a plan word, or read the
+ * applied one.  Changing the plan needs a new pfdd_begin_outer. */
+int pfdd_set_plan(pfdd_ctx *c, int plan);
+int pfdd_get_plan(const pfdd_ctx *c);
+
+/* Autotuner (no reference counterpart): times every candidate plan (fusion
+ * choices x tiles of the fused kernels) over `steps` GL steps of the loaded
+ * problem and applies the fastest; *plan and *ms_per_step (may be NULL)
+ * receive it and its time.  Needs xi uploaded.  xi, tau and the ramp clock
+ * are restored afterwards; pfdd_begin_outer must run again.  Synchronous. */
+int pfdd_autotune(pfdd_ctx *c, int steps, int *plan, float *ms_per_step);
+
 /* Wait for the context's stream. */
 int pfdd_sync(pfdd_ctx *c);
 
@@ -162,6 +182,37 @@ int pfdd_enqueue_stage(pfdd_ctx *c, int stage, int reps);
 int pfdd_step_plan(const pfdd_ctx *c, int *launches);
 
 const char *pfdd_strerror(int rc);
+
+/* ---- batched ensembles ----------------------------------------------------
+ * Many independent PFDD instances (the reference runs one per process).
+ * params[count] holds one parameter set per instance; instances whose
+ * parameters agree in everything (grid, material, slip systems, precision,
+ * B mode, plan) form a class that runs as one batched launch per kernel, and
+ * all classes share one stream and one captured graph per GRAPH_STEPS steps.
+ * Instance indices are 0..count-1 in params order.  The per-instance calls
+ * have the meaning of their single-context counterparts above. */
+typedef struct pfdd_ens pfdd_ens;
+int pfdd_create_ensemble(const pfdd_params *params, int count, pfdd_ens **out);
+void pfdd_destroy_ensemble(pfdd_ens *e);
+/* Number of classes, the class of each instance (class_of[count]) and each
+ * class's plan (plans[nclasses]); the arrays may be NULL. */
+int pfdd_ensemble_info(const pfdd_ens *e, int *nclasses, int *class_of, int *plans);
+/* tau[ns] (uniform) or tau[N1][N2][N3][ns] (per_point) of one instance. */
+int pfdd_ensemble_set_tau(pfdd_ens *e, int inst, const double *tau, int per_point);
+int pfdd_ensemble_upload_xi(pfdd_ens *e, int inst, const double *xi, const double *xi_bc, int nsv);
+int pfdd_ensemble_download_xi(pfdd_ens *e, int inst, double *xi, int nsv);
+/* Prologue of every instance (see pfdd_begin_outer); needs every xi uploaded. */
+int pfdd_ensemble_begin_outer(pfdd_ens *e, int quirk);
+/* nsteps GL steps of every instance; gamma[nsteps][count] (may be NULL)
+ * receives each instance's gamma.  Synchronous when gamma != NULL. */
+int pfdd_ensemble_gl_steps(pfdd_ens *e, int nsteps, double *gamma);
+/* pfdd_autotune for each class; plans[nclasses] and ms_per_step[nclasses]
+ * may be NULL.  pfdd_ensemble_begin_outer must run again afterwards. */
+int pfdd_ensemble_autotune(pfdd_ens *e, int steps, int *plans, float *ms_per_step);
+int pfdd_ensemble_set_plan(pfdd_ens *e, int cls, int plan);
+int pfdd_ensemble_sync(pfdd_ens *e);
+void *pfdd_ensemble_stream(pfdd_ens *e);
+long long pfdd_ensemble_device_bytes(const pfdd_ens *e);
 
 /* ---- host-side setup helpers (pure host code) ---------------------------
  * These restate the reference's setup routines with runtime sizes so that a

@@ -19,6 +19,29 @@ FP64, MIXED = 0, 1
 B_STORED, B_RECOMPUTE = 0, 1
 FORWARD, BACKWARD = -1, 1
 PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS, PLAN_NO_PLANE, PLAN_PLANE = 0, 1, 2, 4, 8, 16
+PLAN_ROW_TILE_SHIFT, PLAN_COL_TILE_SHIFT = 8, 12
+
+
+def plan_row_tile(tile):
+    """Plan bits that fix the fused row kernel's tile (a power of two)."""
+    return (int(tile).bit_length()) << PLAN_ROW_TILE_SHIFT
+
+
+def plan_col_tile(tile):
+    """Plan bits that fix the fused column kernel's tile (a power of two)."""
+    return (int(tile).bit_length()) << PLAN_COL_TILE_SHIFT
+
+
+def describe_plan(plan):
+    names = [n for n, v in (("unfused_rows", 1), ("unfused_cols", 2), ("fused_cols", 4), ("no_plane", 8),
+                            ("plane", 16)) if plan & v]
+    rt, ct = (plan >> PLAN_ROW_TILE_SHIFT) & 15, (plan >> PLAN_COL_TILE_SHIFT) & 15
+    if rt:
+        names.append(f"row_tile={1 << (rt - 1)}")
+    if ct:
+        names.append(f"col_tile={1 << (ct - 1)}")
+    return "+".join(names) if names else "auto"
+
 
 EXPORTED = [
     "pfdd_create", "pfdd_destroy", "pfdd_set_tau", "pfdd_upload_xi", "pfdd_download_xi",
@@ -26,7 +49,11 @@ EXPORTED = [
     "pfdd_device_bytes", "pfdd_strerror", "pfdd_setfcc", "pfdd_setorient", "pfdd_seteps",
     "pfdd_resolve_tau", "pfdd_splitmix64", "pfdd_uniform01", "pfdd_add_loop", "pfdd_enqueue_stage",
     "pfdd_step_plan", "pfdd_seed_loops", "pfdd_seed_sigma", "pfdd_fft_host_init", "fft_forward", "fft_backward",
-    "pfdd_fft_host_status", "pfdd_fft_host_release", "pfdd_set_tau_ramp",
+    "pfdd_fft_host_status", "pfdd_fft_host_release", "pfdd_set_tau_ramp", "pfdd_set_plan", "pfdd_get_plan",
+    "pfdd_autotune", "pfdd_create_ensemble", "pfdd_destroy_ensemble", "pfdd_ensemble_info",
+    "pfdd_ensemble_set_tau", "pfdd_ensemble_upload_xi", "pfdd_ensemble_download_xi", "pfdd_ensemble_begin_outer",
+    "pfdd_ensemble_gl_steps", "pfdd_ensemble_autotune", "pfdd_ensemble_set_plan", "pfdd_ensemble_sync",
+    "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes",
 ]
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
  STAGE_FINISH) = range(8)
@@ -109,6 +136,26 @@ def load():
     L.pfdd_fft_host_status.argtypes = []
     L.pfdd_fft_host_release.argtypes = []
     L.pfdd_fft_host_release.restype = None
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.pfdd_set_plan.argtypes = [vp, ci]
+    L.pfdd_get_plan.argtypes = [vp]
+    L.pfdd_autotune.argtypes = [vp, ci, vp, fp]
+    L.pfdd_create_ensemble.argtypes = [vp, ci, ctypes.POINTER(vp)]
+    L.pfdd_destroy_ensemble.argtypes = [vp]
+    L.pfdd_destroy_ensemble.restype = None
+    L.pfdd_ensemble_info.argtypes = [vp, vp, vp, vp]
+    L.pfdd_ensemble_set_tau.argtypes = [vp, ci, vp, ci]
+    L.pfdd_ensemble_upload_xi.argtypes = [vp, ci, vp, vp, ci]
+    L.pfdd_ensemble_download_xi.argtypes = [vp, ci, vp, ci]
+    L.pfdd_ensemble_begin_outer.argtypes = [vp, ci]
+    L.pfdd_ensemble_gl_steps.argtypes = [vp, ci, vp]
+    L.pfdd_ensemble_autotune.argtypes = [vp, ci, vp, vp]
+    L.pfdd_ensemble_set_plan.argtypes = [vp, ci, ci]
+    L.pfdd_ensemble_sync.argtypes = [vp]
+    L.pfdd_ensemble_stream.argtypes = [vp]
+    L.pfdd_ensemble_stream.restype = vp
+    L.pfdd_ensemble_device_bytes.argtypes = [vp]
+    L.pfdd_ensemble_device_bytes.restype = ctypes.c_longlong
     _lib = L
     return L
 
@@ -233,6 +280,20 @@ class Context:
         _check(load().pfdd_step_plan(self.h, a), "pfdd_step_plan")
         return list(a)
 
+    def set_plan(self, plan):
+        _check(load().pfdd_set_plan(self.h, plan), "pfdd_set_plan")
+
+    @property
+    def plan(self):
+        return load().pfdd_get_plan(self.h)
+
+    def autotune(self, steps=16):
+        """Time the candidate plans, apply the fastest; returns (plan, ms per step)."""
+        plan = ctypes.c_int()
+        ms = ctypes.c_float()
+        _check(load().pfdd_autotune(self.h, steps, ctypes.byref(plan), ctypes.byref(ms)), "pfdd_autotune")
+        return plan.value, ms.value
+
     @property
     def stream(self):
         return load().pfdd_stream(self.h)
@@ -240,3 +301,96 @@ class Context:
     @property
     def device_bytes(self):
         return load().pfdd_device_bytes(self.h)
+
+
+class Ensemble:
+    """One pfdd_ens: many independent instances, grouped into batched classes."""
+
+    def __init__(self, params_list):
+        L = load()
+        self.params = list(params_list)
+        self.count = len(self.params)
+        arr = (Params * self.count)(*self.params)
+        self._arr = arr
+        h = ctypes.c_void_p()
+        _check(L.pfdd_create_ensemble(arr, self.count, ctypes.byref(h)), "pfdd_create_ensemble")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pfdd_destroy_ensemble(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        """(number of classes, class of each instance, plan of each class)."""
+        n = ctypes.c_int()
+        _check(load().pfdd_ensemble_info(self.h, ctypes.byref(n), None, None), "pfdd_ensemble_info")
+        cls = np.zeros(self.count, dtype=np.int32)
+        plans = np.zeros(n.value, dtype=np.int32)
+        _check(load().pfdd_ensemble_info(self.h, None, cls.ctypes.data, plans.ctypes.data), "pfdd_ensemble_info")
+        return n.value, cls, plans
+
+    def set_tau(self, inst, tau, per_point=False):
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        _check(load().pfdd_ensemble_set_tau(self.h, inst, t.ctypes.data, 1 if per_point else 0),
+               "pfdd_ensemble_set_tau")
+
+    def upload_xi(self, inst, xi, xi_bc=None):
+        x = np.ascontiguousarray(xi, dtype=np.complex128)
+        b = None if xi_bc is None else np.ascontiguousarray(xi_bc, dtype=np.complex128)
+        _check(load().pfdd_ensemble_upload_xi(self.h, inst, x.ctypes.data, None if b is None else b.ctypes.data,
+                                              x.shape[0]), "pfdd_ensemble_upload_xi")
+
+    def download_xi(self, inst, nsv=None):
+        p = self.params[inst]
+        nsv = p.ns if nsv is None else nsv
+        x = np.zeros((nsv, p.n1, p.n2, p.n3), dtype=np.complex128)
+        _check(load().pfdd_ensemble_download_xi(self.h, inst, x.ctypes.data, nsv), "pfdd_ensemble_download_xi")
+        return x
+
+    def begin_outer(self, quirk=True):
+        _check(load().pfdd_ensemble_begin_outer(self.h, 1 if quirk else 0), "pfdd_ensemble_begin_outer")
+
+    def gl_steps(self, n, record=True):
+        """n GL steps of every instance; returns gamma [n][count] (or None)."""
+        if not record:
+            _check(load().pfdd_ensemble_gl_steps(self.h, n, None), "pfdd_ensemble_gl_steps")
+            return None
+        g = np.zeros((n, self.count))
+        _check(load().pfdd_ensemble_gl_steps(self.h, n, g.ctypes.data), "pfdd_ensemble_gl_steps")
+        return g
+
+    def autotune(self, steps=16):
+        """Per class: (plan, ms per step) of the fastest candidate, which is applied."""
+        n = self.info()[0]
+        plans = np.zeros(n, dtype=np.int32)
+        ms = np.zeros(n, dtype=np.float32)
+        _check(load().pfdd_ensemble_autotune(self.h, steps, plans.ctypes.data, ms.ctypes.data),
+               "pfdd_ensemble_autotune")
+        return plans, ms
+
+    def set_plan(self, cls, plan):
+        _check(load().pfdd_ensemble_set_plan(self.h, cls, plan), "pfdd_ensemble_set_plan")
+
+    def sync(self):
+        _check(load().pfdd_ensemble_sync(self.h), "pfdd_ensemble_sync")
+
+    @property
+    def stream(self):
+        return load().pfdd_ensemble_stream(self.h)
+
+    @property
+    def device_bytes(self):
+        return load().pfdd_ensemble_device_bytes(self.h)

@@ -236,4 +236,82 @@ def small_fcc(shape, nloops=4, seed=BASE_SEED + 100, ns=12, rmin=2.0, rmax=6.0):
     return Workload(f"fcc{shape}", tuple(shape), eps, normals, xi, resolve_tau(sigma, eps), sigma, 10, loops=loops)
 
 
+C3_SIZES = (32, 48, 64, 96)
+
+
+@dataclass
+class Member:
+    """One instance of an ensemble workload."""
+    index: int
+    shape: tuple
+    xi: np.ndarray           # complex [ns, N, N, N]
+    tau: np.ndarray          # [ns]
+    sigma: np.ndarray        # [3,3]
+    loops: list
+
+    @property
+    def P(self):
+        return int(np.prod(self.shape))
+
+
+class EnsembleWorkload:
+    """Independent NS=3 instances (the systems of one (111) plane, as c0).
+    Instance i draws from SplitMix64 seeded BASE_SEED + 1000 + i: the grid
+    edge from `sizes` (one draw), the loop count 1..4 (one draw), the loops
+    (r ~ U[4,15], system, centre), then the resolved shear tau ~ U[0, 0.05]
+    of system 0, applied as sigma = tau * sym(e_0)/(sym(e_0):e_0).
+    Members are generated on demand, so the whole ensemble never sits in
+    host memory at once."""
+
+    def __init__(self, name, count, steps, sizes=C3_SIZES, rmin=4.0, rmax=15.0, tau_max=0.05, first=0):
+        self.name, self.count, self.steps = name, count, steps
+        self.sizes, self.rmin, self.rmax, self.tau_max, self.first = tuple(sizes), rmin, rmax, tau_max, first
+        theta = setorient(ROWS_111_TO_Z)
+        self.eps, self.normals = _fcc_systems([0, 1, 2], theta)
+        e0 = self.eps[0]
+        se = 0.5 * (e0 + e0.T)
+        self.s_hat = se / np.sum(se * e0)
+        self._edges = [self._edge(Rng(self.seed(i))) for i in range(count)]
+
+    def seed(self, i):
+        return BASE_SEED + 1000 + self.first + i
+
+    def _edge(self, rng):
+        return self.sizes[min(int(rng.uniform() * len(self.sizes)), len(self.sizes) - 1)]
+
+    @property
+    def ns(self):
+        return 3
+
+    def shape(self, i):
+        n = self._edges[i]
+        return (n, n, n)
+
+    def total_points(self):
+        return sum(n ** 3 for n in self._edges)
+
+    def member(self, i):
+        rng = Rng(self.seed(i))
+        n = self._edge(rng)
+        nloops = 1 + min(int(rng.uniform() * 4), 3)
+        xi = np.zeros((3, n, n, n), dtype=np.complex128)
+        loops = _random_loops(rng, xi, self.normals, nloops, self.rmin, self.rmax)
+        tau0 = self.tau_max * rng.uniform()
+        sigma = tau0 * self.s_hat
+        return Member(i, (n, n, n), xi, resolve_tau(sigma, self.eps), sigma, loops)
+
+    def params(self, i, precision=MIXED, bmode=B_RECOMPUTE, plan=0):
+        return make_params(self.shape(i), self.eps, C11, C12, C44, [0.1] * 3, asf2_reference(3), [1.0] * 3,
+                           precision=precision, bmode=bmode, pi_const=PI_REF, plan=plan)
+
+    def params_list(self, precision=MIXED, bmode=B_RECOMPUTE, plan=0):
+        return [self.params(i, precision, bmode, plan) for i in range(self.count)]
+
+
+def c3(count=512, steps=500, sizes=C3_SIZES):
+    """512 independent NS=3 instances on grids drawn from {32,48,64,96}^3,
+    1-4 loops of radius 4-15 each, tau ~ U[0,0.05] mu; 500 GL steps."""
+    return EnsembleWorkload("c3", count, steps, sizes)
+
+
 BY_NAME = {"c0": c0, "c1": c1, "c2": c2, "c4": c4}

@@ -3,7 +3,10 @@
 // One pfdd_ctx owns its device memory and one HIP stream (default priority).
 // A GL step is: k_update (plasticevolv + copy to the FFT array + gamma sums),
 // forward 3D FFT, contraction with B(k), inverse 3D FFT.  Chunks of steps
-// are captured once into a hipGraph and replayed.
+// are captured once into a hipGraph and replayed.  A context may hold B
+// independent instances of one grid (the classes of an ensemble): every
+// kernel then runs over the B*S fields, and only the contraction couples the
+// S fields of one instance.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,19 +45,23 @@ struct pfdd_ctx {
   pfdd_params p{};
   long P = 0;
   int S = 0;
+  int B = 1;                 // instances of this grid
+  int F = 0;                 // fields, B * S
   int ntri = 0;
   int fp64 = 1;
   int stored = 0;
   hipStream_t stream = nullptr;
+  int owns_stream = 1;       // 0: the stream belongs to an ensemble
   // device buffers
   double2* xi = nullptr;
   void* W = nullptr;         // cx<T> [S][P]
   void* Bsym = nullptr;      // T [ntri][P]
   uint8_t* mask = nullptr;
-  double* taupp = nullptr;
+  double* taupp = nullptr;   // [B][P][S], when some instance has a per-point shear
+  double2* taud = nullptr;   // [F] (tau, ramp rate)
   double* partials = nullptr;
   size_t partials_len = 0;
-  double* hist = nullptr;    // [HIST_CAP][S+1]
+  double* hist = nullptr;    // [HIST_CAP][B][S+1]
   int* step_ctr = nullptr;
   double* small = nullptr;   // reduction outputs (<= 32 doubles)
   double* avepst_d = nullptr;
@@ -71,10 +78,14 @@ struct pfdd_ctx {
   int c3_tile = 0;           // columns per block of the fused column kernel (0: not fused)
   int plane = 0;             // 1: the LDS-resident plane kernel does the N2/N3 passes and the update
   size_t c3_lds = 0;         // its dynamic LDS bytes
+  int plan = 0;              // the applied PFDD_PLAN_* word
   int has_xi = 0;
   int prologue_done = 0;
   hipGraphExec_t graph = nullptr;
   long long bytes = 0;
+  // per instance
+  std::vector<double2> tauh; // host copy of taud
+  std::vector<uint8_t> inst_pp, inst_frozen, inst_up;
 };
 
 namespace {
@@ -134,14 +145,14 @@ int fft3d(pfdd_ctx* c, cx<T>* data, int batch, int sign) {
 
 template <typename T>
 int contract(pfdd_ctx* c, int quirk) {
-  const unsigned nb = (unsigned)((c->P + 255) / 256);
+  const dim3 grid((unsigned)((c->P + 255) / 256), (unsigned)c->B);
   cx<T>* W = (cx<T>*)c->W;
   const T* B = (const T*)c->Bsym;
 #define PFDD_CT(SM)                                                                                     \
   if (c->stored)                                                                                        \
-    hipLaunchKernelGGL((k_contract<T, SM, true>), dim3(nb), dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
+    hipLaunchKernelGGL((k_contract<T, SM, true>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
   else                                                                                                  \
-    hipLaunchKernelGGL((k_contract<T, SM, false>), dim3(nb), dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
+    hipLaunchKernelGGL((k_contract<T, SM, false>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
   if (c->S <= 4) { PFDD_CT(4) } else if (c->S <= 8) { PFDD_CT(8) } else if (c->S <= 12) { PFDD_CT(12) } else { PFDD_CT(16) }
 #undef PFDD_CT
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
@@ -149,7 +160,7 @@ int contract(pfdd_ctx* c, int quirk) {
 
 template <typename T>
 int update(pfdd_ctx* c, bool finish = true) {
-  dim3 grid((unsigned)c->upd_nblk, (unsigned)c->S);
+  dim3 grid((unsigned)c->upd_nblk, (unsigned)c->F);
   cx<T>* W = (cx<T>*)c->W;
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
 #define PFDD_UP(MM, TT)                                                                                  \
@@ -161,7 +172,7 @@ int update(pfdd_ctx* c, bool finish = true) {
   else PFDD_UP(false, false);
 #undef PFDD_UP
   if (finish)
-    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->upd_nblk,
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->upd_nblk, c->S,
                        c->hist, c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
@@ -192,7 +203,7 @@ int row_tile_max(int n, bool fp64) {
 template <typename T, int N>
 void k1_launch_n(pfdd_ctx* c) {
   cx<T>* W = (cx<T>*)c->W;
-  const long nb = (long)c->S * c->row_nblk;
+  const long nb = (long)c->F * c->row_nblk;
   constexpr int NT = tile_rows<T, N>() * make_plan(N).tpp;
   const cx<T>* tw = tw_of<T>(c->tw[2]);
 #define PFDD_K1(MM, TT)                                                                                   \
@@ -215,7 +226,7 @@ int k1_launch(pfdd_ctx* c, bool finish = true) {
     default: return PFDD_E_RADIX;
   }
   if (finish)
-    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->row_nblk,
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->row_nblk, c->S,
                        c->hist, c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
@@ -234,8 +245,8 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
   if (attr_only) {                                                                                              \
     HIPCK(hipFuncSetAttribute(k3_symbol<T, N, SM>(), hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->c3_lds)); \
   } else {                                                                                                      \
-    hipLaunchKernelGGL((k_col_contract<T, N, SM>), dim3((unsigned)nb), dim3(C3_THREADS), c->c3_lds, c->stream, W,      \
-                       c->ec, c->P, c->c3_tile, quirk, tw);                                                     \
+    hipLaunchKernelGGL((k_col_contract<T, N, SM>), dim3((unsigned)nb, (unsigned)c->B), dim3(C3_THREADS), c->c3_lds, \
+                       c->stream, W, c->ec, c->P, c->c3_tile, quirk, tw);                                       \
   }
   if (c->S <= 4) { PFDD_K3(4) } else if (c->S <= 8) { PFDD_K3(8) } else if (c->S <= 12) { PFDD_K3(12) } else { PFDD_K3(16) }
 #undef PFDD_K3
@@ -263,7 +274,7 @@ int plane_launch_n(pfdd_ctx* c, bool attr_only) {
   constexpr size_t lds = plane_lds_bytes<T, N, N>();
   const cx<T>* tw2 = tw_of<T>(c->tw[1]);
   const cx<T>* tw3 = tw_of<T>(c->tw[2]);
-  const unsigned nb = (unsigned)(c->S * c->p.n1);
+  const unsigned nb = (unsigned)(c->F * c->p.n1);
   cx<T>* W = (cx<T>*)c->W;
 #define PFDD_PL(MM, TT)                                                                                          \
   if (attr_only) {                                                                                               \
@@ -304,8 +315,8 @@ int plane_launch(pfdd_ctx* c, bool finish = true, bool attr_only = false) {
   }
   if (rc || attr_only) return rc;
   if (finish)
-    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S, c->p.n1, c->hist,
-                       c->step_ctr);
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->p.n1, c->S,
+                       c->hist, c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -315,11 +326,11 @@ template <typename T>
 int n1_part(pfdd_ctx* c, int quirk) {
   int rc;
   cx<T>* W = (cx<T>*)c->W;
-  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
   if (c->c3_tile > 0) return k3_launch<T>(c, quirk);
-  if ((rc = launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_FORWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
+  if ((rc = launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_FORWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
   if ((rc = contract<T>(c, quirk))) return rc;
-  return launch_axis<T>(W, S, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream);
+  return launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream);
 }
 
 // Passes between the forward N3 pass and the inverse N3 pass: forward N2,
@@ -329,15 +340,15 @@ template <typename T>
 int middle(pfdd_ctx* c, int quirk) {
   int rc;
   cx<T>* W = (cx<T>*)c->W;
-  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
-  if ((rc = launch_axis<T>(W, (long)S * n1, n2, n3, PFDD_FORWARD, tw_of<T>(c->tw[1]), c->stream))) return rc;
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
+  if ((rc = launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_FORWARD, tw_of<T>(c->tw[1]), c->stream))) return rc;
   if ((rc = n1_part<T>(c, quirk))) return rc;
-  return launch_axis<T>(W, (long)S * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream);
+  return launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream);
 }
 
 template <typename T>
 int axis3(pfdd_ctx* c, int sign) {
-  return launch_axis<T>((cx<T>*)c->W, (long)c->S * c->p.n1 * c->p.n2, c->p.n3, 1, sign, tw_of<T>(c->tw[2]),
+  return launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1 * c->p.n2, c->p.n3, 1, sign, tw_of<T>(c->tw[2]),
                         c->stream);
 }
 
@@ -365,11 +376,11 @@ int one_step(pfdd_ctx* c) {
 template <typename T>
 int prologue(pfdd_ctx* c, int quirk) {
   int rc;
-  const long n = (long)c->S * c->P;
+  const long n = (long)c->F * c->P;
   hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, (cx<T>*)c->W, n);
   if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
   if (c->plane) {  // the next step's plane kernel finishes the N2 and N3 inverse passes
-    if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->S * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD, tw_of<T>(c->tw[1]),
+    if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD, tw_of<T>(c->tw[1]),
                              c->stream)))
       return rc;
     return n1_part<T>(c, quirk);
@@ -448,8 +459,8 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
   {
     dim3 grid((unsigned)c->upd_nblk, (unsigned)S);
     hipLaunchKernelGGL(k_xi_sums, grid, dim3(UPD_THREADS), 0, c->stream, c->xi, P, c->upd_nblk, c->partials);
-    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, S, c->upd_nblk, c->small,
-                       (int*)nullptr);
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, S, c->upd_nblk, S,
+                       c->small, (int*)nullptr);
     HIPCK(hipPeekAtLastError());
   }
   double sums[MAXS + 1];
@@ -480,7 +491,7 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
                      c->partials, (int)nb, ef, sf)
   if (S <= 4) PFDD_SR(4); else if (S <= 8) PFDD_SR(8); else PFDD_SR(16);
 #undef PFDD_SR
-  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, 18, (int)nb, c->small,
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, 18, (int)nb, 18, c->small,
                      (int*)nullptr);
   HIPCK(hipPeekAtLastError());
   double out[19];
@@ -519,21 +530,102 @@ void pfdd_destroy(pfdd_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
-  void* bufs[] = {c->xi, c->W, c->Bsym, c->mask, c->taupp, c->partials, c->hist, c->step_ctr,
-                  c->small, c->avepst_d, c->ec, c->E6, c->efield, c->sfield, c->freq};
+  void* bufs[] = {c->xi,   c->W,        c->Bsym, c->mask, c->taupp, c->taud,   c->partials, c->hist,
+                  c->step_ctr, c->small, c->avepst_d, c->ec, c->E6,  c->efield, c->sfield,   c->freq};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& t : c->tw) {
     if (t.d) (void)hipFree(t.d);
     if (t.f) (void)hipFree(t.f);
   }
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
-int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
-  if (!p || !out) return PFDD_E_SHAPE;
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+constexpr int PLAN_FLAGS = PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS |
+                           PFDD_PLAN_NO_PLANE | PFDD_PLAN_PLANE;
+constexpr int PLAN_TILES = (15 << PFDD_PLAN_ROW_TILE_SHIFT) | (15 << PFDD_PLAN_COL_TILE_SHIFT);
+
+void drop_graph(pfdd_ctx* c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+}
+
+// Kernel choice for a plan word (PFDD_E_SHAPE when a requested tile does not
+// fit).  The state of W between steps depends on the kernels, so a new
+// prologue is needed afterwards.
+int apply_plan(pfdd_ctx* c, int plan) {
+  const pfdd_params& p = c->p;
+  if (plan & ~(PLAN_FLAGS | PLAN_TILES)) return PFDD_E_SHAPE;
+  const int S = c->S;
+  const size_t csz = c->fp64 ? sizeof(double2) : sizeof(float2);
+  const int rreq = (plan >> PFDD_PLAN_ROW_TILE_SHIFT) & 15, creq = (plan >> PFDD_PLAN_COL_TILE_SHIFT) & 15;
+  int row_tile = 0, row_nblk = 0, c3_tile = 0, plane = 0;
+  size_t c3_lds = 0;
+  // fused row kernel: rows per block = the largest power of two <= its
+  // compile-time tile that divides the N1*N2 rows of a field (or the one asked for)
+  if (fused_len(p.n3) && !(plan & PFDD_PLAN_UNFUSED_ROWS)) {
+    const long rows = (long)p.n1 * p.n2;
+    const int tmax = row_tile_max(p.n3, c->fp64);
+    int t = 1;
+    if (rreq) {
+      t = 1 << (rreq - 1);
+      if (t > tmax || rows % t) return PFDD_E_SHAPE;
+    } else {
+      while (t * 2 <= tmax && rows % (t * 2) == 0) t *= 2;
+    }
+    row_tile = t;
+    row_nblk = (int)(rows / t);
+  } else if (rreq) {
+    return PFDD_E_SHAPE;
+  }
+  // fused column kernel: S*tile pencils of N1 points in LDS
+  if (fused_len(p.n1) && !c->stored && !(plan & PFDD_PLAN_UNFUSED_COLS)) {
+    const size_t per_col = (size_t)S * (p.n1 + 1) * csz;
+    const int maxpen = (c->fp64 ? 1 : 2) * (C3_THREADS / make_plan(p.n1).tpp);
+    const long inner = (long)p.n2 * p.n3;
+    auto fits = [&](int t) { return t * S <= maxpen && t * per_col <= C3_LDS_BUDGET && inner % t == 0; };
+    int t = 0;
+    if (creq) {
+      t = 1 << (creq - 1);
+      if (!fits(t)) return PFDD_E_SHAPE;
+    } else {
+      for (int cand = 1; fits(cand); cand *= 2) t = cand;
+      if (t < ((plan & PFDD_PLAN_FUSED_COLS) ? 1 : C3_MIN_TILE)) t = 0;
+    }
+    if (t > 0) {
+      c3_tile = t;
+      c3_lds = t * per_col;
+    }
+  } else if (creq) {
+    return PFDD_E_SHAPE;
+  }
+  // the plane kernel holds one block per CU; at 128^2 (FP32) it measured no faster than the row kernel +
+  // two N2 passes (0.36 vs 0.35 ms at 128^3 x 12), so AUTO takes it up to 96^2 and PFDD_PLAN_PLANE forces it
+  if (!(plan & PFDD_PLAN_NO_PLANE) && p.n2 == p.n3 && plane_len(p.n3, c->fp64) &&
+      (p.n3 <= 96 || (plan & PFDD_PLAN_PLANE)))
+    plane = 1;
+  c->row_tile = row_tile;
+  c->row_nblk = row_nblk;
+  c->c3_tile = c3_tile;
+  c->c3_lds = c3_lds;
+  c->plane = plane;
+  int rc = PFDD_OK;
+  if (c3_tile > 0 && (rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true))) return rc;
+  if (plane && (rc = c->fp64 ? plane_launch<double>(c, false, true) : plane_launch<float>(c, false, true))) return rc;
+  drop_graph(c);
+  c->plan = plan;
+  c->prologue_done = 0;
+  return PFDD_OK;
+}
+
+int validate_params(const pfdd_params* p) {
   if (p->ns < 1 || p->ns > PFDD_MAX_NS) return PFDD_E_SHAPE;
   if (p->precision != PFDD_FP64 && p->precision != PFDD_MIXED) return PFDD_E_SHAPE;
   if (p->bmode != PFDD_B_STORED && p->bmode != PFDD_B_RECOMPUTE) return PFDD_E_SHAPE;
@@ -550,67 +642,68 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     for (int i = 0; i < 9; i++)
       if (!std::isfinite(p->eps[a][i / 3][i % 3])) return PFDD_E_NAN;
   }
+  if (p->plan & ~(PLAN_FLAGS | PLAN_TILES)) return PFDD_E_SHAPE;
+  return PFDD_OK;
+}
+
+int push_tau(pfdd_ctx* c) {
+  HIPCK(hipMemcpyAsync(c->taud, c->tauh.data(), sizeof(double2) * c->F, hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return PFDD_OK;
+}
+
+// A context of `batch` instances of the grid in *p.  With `shared` the
+// context runs on that stream (an ensemble's) and does not own it.
+int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** out) {
+  *out = nullptr;
+  int rc = validate_params(p);
+  if (rc) return rc;
+  if (batch < 1 || (long)batch * p->ns > FIN_MAX_FIELDS) return PFDD_E_SHAPE;
   pfdd_ctx* c = new (std::nothrow) pfdd_ctx();
   if (!c) return PFDD_E_ALLOC;
   c->p = *p;
   c->S = p->ns;
+  c->B = batch;
+  c->F = batch * p->ns;
   c->P = (long)p->n1 * p->n2 * p->n3;
   c->ntri = c->S * (c->S + 1) / 2;
   c->fp64 = p->precision == PFDD_FP64;
   c->stored = p->bmode == PFDD_B_STORED;
-  int rc = PFDD_OK;
+  c->tauh.assign((size_t)c->F, make_double2(0.0, 0.0));
+  c->inst_pp.assign((size_t)batch, 0);
+  c->inst_frozen.assign((size_t)batch, 0);
+  c->inst_up.assign((size_t)batch, 0);
 #define FAIL(code) do { rc = (code); goto fail; } while (0)
   {
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (shared) {
+      c->stream = shared;
+      c->owns_stream = 0;
+    } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      FAIL(PFDD_E_HIP);
+    }
     const size_t csz = c->fp64 ? sizeof(double2) : sizeof(float2);
     const size_t rsz = c->fp64 ? sizeof(double) : sizeof(float);
     const long P = c->P;
-    const int S = c->S;
+    const int S = c->S, F = c->F;
+    const int ns3[3] = {p->n1, p->n2, p->n3};
     c->upd_nblk = (int)((P + UPD_THREADS * UPD_EPT - 1) / (UPD_THREADS * UPD_EPT));
     const long strain_nblk = (P + 255) / 256;
-    // fused row kernel: rows per block = the largest power of two <= its
-    // compile-time tile that divides the N1*N2 rows of a field
-    if (p->plan & ~(PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS | PFDD_PLAN_NO_PLANE |
-                    PFDD_PLAN_PLANE))
-      FAIL(PFDD_E_SHAPE);
-    if (fused_len(p->n3) && !(p->plan & PFDD_PLAN_UNFUSED_ROWS)) {
-      const long rows = (long)p->n1 * p->n2;
-      const int tmax = row_tile_max(p->n3, c->fp64);
-      int t = 1;
-      while (t * 2 <= tmax && rows % (t * 2) == 0) t *= 2;
-      c->row_tile = t;
-      c->row_nblk = (int)(rows / t);
-    }
-    // fused column kernel: S*tile pencils of N1 points in LDS
-    if (fused_len(p->n1) && !c->stored && !(p->plan & PFDD_PLAN_UNFUSED_COLS)) {
-      const size_t per_col = (size_t)S * (p->n1 + 1) * csz;
-      const int maxpen = (c->fp64 ? 1 : 2) * (C3_THREADS / make_plan(p->n1).tpp);
-      const long inner = (long)p->n2 * p->n3;
-      int t = 0;
-      for (int cand = 1; cand * S <= maxpen && cand * per_col <= C3_LDS_BUDGET && inner % cand == 0; cand *= 2) t = cand;
-      if (t >= ((p->plan & PFDD_PLAN_FUSED_COLS) ? 1 : C3_MIN_TILE)) {
-        c->c3_tile = t;
-        c->c3_lds = t * per_col;
-      }
-    }
-    // the plane kernel holds one block per CU; at 128^2 (FP32) it measured no faster than the row kernel +
-    // two N2 passes (0.36 vs 0.35 ms at 128^3 x 12), so AUTO takes it up to 96^2 and PFDD_PLAN_PLANE forces it
-    if (!(p->plan & PFDD_PLAN_NO_PLANE) && p->n2 == p->n3 && plane_len(p->n3, c->fp64) &&
-        (p->n3 <= 96 || (p->plan & PFDD_PLAN_PLANE)))
-      c->plane = 1;
-    c->partials_len = (size_t)std::max<long>(std::max<long>((long)S * c->upd_nblk, (long)S * c->row_nblk),
-                                             std::max<long>(18 * strain_nblk, (long)S * p->n1));
-    if (!(c->xi = (double2*)dev_alloc<char>(sizeof(double2) * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
-    if (!(c->W = dev_alloc<char>(csz * S * P, c->bytes))) FAIL(PFDD_E_ALLOC);
+    // block sums of every plan (the row kernel's tile may go down to one row)
+    c->partials_len = (size_t)std::max<long>(std::max<long>((long)F * c->upd_nblk, 18 * strain_nblk),
+                                             std::max<long>((long)F * p->n1 * p->n2, (long)F * p->n1));
+    if (!(c->xi = (double2*)dev_alloc<char>(sizeof(double2) * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->W = dev_alloc<char>(csz * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
-    if (!(c->hist = (double*)dev_alloc<char>(sizeof(double) * HIST_CAP * (S + 1), c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->hist = (double*)dev_alloc<char>(sizeof(double) * HIST_CAP * c->B * (S + 1), c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->step_ctr = (int*)dev_alloc<char>(sizeof(int) * 4, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->small = (double*)dev_alloc<char>(sizeof(double) * 32, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->avepst_d = (double*)dev_alloc<char>(sizeof(double) * 9, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->ec = (ElasticConsts*)dev_alloc<char>(sizeof(ElasticConsts), c->bytes))) FAIL(PFDD_E_ALLOC);
+    if (!(c->taud = (double2*)dev_alloc<char>(sizeof(double2) * F, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (hipMemsetAsync(c->step_ctr, 0, sizeof(int) * 4, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
-    if (hipMemsetAsync(c->xi, 0, sizeof(double2) * S * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
-    if (hipMemsetAsync(c->W, 0, csz * S * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipMemsetAsync(c->xi, 0, sizeof(double2) * F * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipMemsetAsync(c->W, 0, csz * F * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if ((rc = push_tau(c))) goto fail;
     // twiddle tables, one per axis
     for (int i = 0; i < 3; i++)
       if ((rc = make_twiddles(c->tw[i], ns3[i], c->bytes, c->stream))) goto fail;
@@ -648,12 +741,11 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
     std::vector<double> fr((size_t)p->n1 + p->n2 + p->n3);
     {
       size_t o = 0;
-      const int nn[3] = {p->n1, p->n2, p->n3};
       for (int ax = 0; ax < 3; ax++)
-        for (int i = 0; i < nn[ax]; i++, o++) {
+        for (int i = 0; i < ns3[ax]; i++, o++) {
           const double dd = h.d[ax];
-          fr[o] = i == 0 ? 0.0 : i < nn[ax] / 2 ? (double)i / (double)nn[ax] / dd
-                                                 : ((double)i - (double)nn[ax]) / (double)nn[ax] / dd;
+          fr[o] = i == 0 ? 0.0 : i < ns3[ax] / 2 ? (double)i / (double)ns3[ax] / dd
+                                                  : ((double)i - (double)ns3[ax]) / (double)ns3[ax] / dd;
         }
     }
     if (!(c->freq = (double*)dev_alloc<char>(sizeof(double) * fr.size(), c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -678,8 +770,8 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
       c->sa.cd2[a] = p->cd2[a];
       c->sa.asf2[a] = p->asf2[a];
       c->sa.dslip[a] = p->dslip[a];
-      c->sa.tau[a] = 0.0;
     }
+    c->sa.tau = c->taud;
     if (c->stored) {
       if (!(c->Bsym = dev_alloc<char>(rsz * c->ntri * P, c->bytes))) FAIL(PFDD_E_ALLOC);
       const unsigned nb = (unsigned)((P + 255) / 256);
@@ -692,14 +784,7 @@ int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
 #undef PFDD_BB
       if (hipPeekAtLastError() != hipSuccess) FAIL(PFDD_E_HIP);
     }
-    if (c->c3_tile > 0) {
-      rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true);
-      if (rc) goto fail;
-    }
-    if (c->plane) {
-      rc = c->fp64 ? plane_launch<double>(c, false, true) : plane_launch<float>(c, false, true);
-      if (rc) goto fail;
-    }
+    if ((rc = apply_plan(c, p->plan))) goto fail;
     if (hipStreamSynchronize(c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
   }
 #undef FAIL
@@ -710,96 +795,117 @@ fail:
   return rc;
 }
 
-int pfdd_set_tau_ramp(pfdd_ctx* c, const double* tau0, const double* rate) {
-  if (!c || !tau0 || !rate) return PFDD_E_SHAPE;
-  for (int a = 0; a < c->S; a++)
-    if (!std::isfinite(tau0[a]) || !std::isfinite(rate[a])) return PFDD_E_NAN;
-  int rc = pfdd_set_tau(c, tau0, 0);
-  if (rc) return rc;
-  for (int a = 0; a < c->S; a++) c->sa.tau_rate[a] = rate[a];
-  return PFDD_OK;
-}
-
-int pfdd_set_tau(pfdd_ctx* c, const double* tau, int per_point) {
-  if (!c || !tau) return PFDD_E_SHAPE;
+// Resolved shear of instance b: tau[S] (uniform) or tau[P][S] (per point).
+// Rates are reset to zero.  While any instance has a per-point shear the
+// kernels read taupp for every instance, so uniform ones are expanded there.
+int set_tau_inst(pfdd_ctx* c, int b, const double* tau, int per_point) {
   const int S = c->S;
-  for (int a = 0; a < S; a++) c->sa.tau_rate[a] = 0.0;
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr + 1, 0);
-  HIPCK(hipPeekAtLastError());
+  const long P = c->P;
+  const size_t n = (size_t)P * S;
   if (!per_point) {
-    for (int a = 0; a < S; a++) {
+    for (int a = 0; a < S; a++)
       if (!std::isfinite(tau[a])) return PFDD_E_NAN;
-      c->sa.tau[a] = tau[a];
-    }
+  } else {
+    for (size_t i = 0; i < n; i++)
+      if (!std::isfinite(tau[i])) return PFDD_E_NAN;
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->inst_pp[b] = per_point ? 1 : 0;
+  for (int a = 0; a < S; a++) c->tauh[(size_t)b * S + a] = make_double2(per_point ? 0.0 : tau[a], 0.0);
+  bool any_pp = false;
+  for (uint8_t v : c->inst_pp) any_pp = any_pp || v;
+  if (!any_pp) {
     if (c->taupp) {
-      HIPCK(hipStreamSynchronize(c->stream));
       (void)hipFree(c->taupp);
       c->taupp = nullptr;
     }
   } else {
-    const size_t n = (size_t)c->P * S;
-    for (size_t i = 0; i < n; i++)
-      if (!std::isfinite(tau[i])) return PFDD_E_NAN;
-    if (!c->taupp && !(c->taupp = (double*)dev_alloc<char>(sizeof(double) * n, c->bytes))) return PFDD_E_ALLOC;
-    HIPCK(hipMemcpyAsync(c->taupp, tau, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCK(hipStreamSynchronize(c->stream));
+    std::vector<double> h;
+    auto expand = [&](int j) -> int {  // uniform shear of instance j -> its taupp slice
+      h.resize(n);
+      for (long pt = 0; pt < P; pt++)
+        for (int a = 0; a < S; a++) h[(size_t)pt * S + a] = c->tauh[(size_t)j * S + a].x;
+      HIPCK(hipMemcpyAsync(c->taupp + (size_t)j * n, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+      HIPCK(hipStreamSynchronize(c->stream));
+      return PFDD_OK;
+    };
+    int rc;
+    if (!c->taupp) {
+      if (!(c->taupp = (double*)dev_alloc<char>(sizeof(double) * n * c->B, c->bytes))) return PFDD_E_ALLOC;
+      for (int j = 0; j < c->B; j++)
+        if (j != b && !c->inst_pp[j] && (rc = expand(j))) return rc;
+    }
+    if (per_point) {
+      HIPCK(hipMemcpyAsync(c->taupp + (size_t)b * n, tau, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+      HIPCK(hipStreamSynchronize(c->stream));
+    } else if ((rc = expand(b))) {
+      return rc;
+    }
   }
-  // the captured graph holds the old kernel arguments
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  return PFDD_OK;
+  drop_graph(c);  // the captured graph holds the old kernel arguments
+  return push_tau(c);
 }
 
-int pfdd_upload_xi(pfdd_ctx* c, const double* xi, const double* xi_bc, int nsv) {
-  if (!c || !xi || nsv < c->S) return PFDD_E_SHAPE;
+int upload_inst(pfdd_ctx* c, int b, const double* xi, const double* xi_bc, int nsv) {
+  if (!xi || nsv < c->S) return PFDD_E_SHAPE;
   const long P = c->P;
   const int S = c->S;
-  for (long i = 0; i < 2 * (long)S * P; i++)
+  const size_t n = (size_t)S * P, off = (size_t)b * n;
+  for (size_t i = 0; i < 2 * n; i++)
     if (!std::isfinite(xi[i])) return PFDD_E_NAN;
-  for (long i = 2 * (long)S * P; i < 2 * (long)nsv * P; i++)
+  for (size_t i = 2 * n; i < 2 * (size_t)nsv * P; i++)
     if (xi[i] != 0.0) return PFDD_E_UNSUPPORTED;
-  bool any_frozen = false;
+  bool frozen = false;
   std::vector<uint8_t> m;
   if (xi_bc) {
-    for (long i = 0; i < (long)S * P && !any_frozen; i++) any_frozen = xi_bc[2 * i] != 0.0;
-    if (any_frozen) {
-      m.resize((size_t)S * P);
-      for (long i = 0; i < (long)S * P; i++) m[i] = xi_bc[2 * i] != 0.0;
+    for (size_t i = 0; i < n && !frozen; i++) frozen = xi_bc[2 * i] != 0.0;
+    if (frozen) {
+      m.resize(n);
+      for (size_t i = 0; i < n; i++) m[i] = xi_bc[2 * i] != 0.0;
     }
   }
   HIPCK(hipStreamSynchronize(c->stream));
-  HIPCK(hipMemcpyAsync(c->xi, xi, sizeof(double2) * S * P, hipMemcpyHostToDevice, c->stream));
-  if (any_frozen) {
-    if (!c->mask && !(c->mask = (uint8_t*)dev_alloc<char>((size_t)S * P, c->bytes))) return PFDD_E_ALLOC;
-    HIPCK(hipMemcpyAsync(c->mask, m.data(), (size_t)S * P, hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipMemcpyAsync(c->xi + off, xi, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+  c->inst_frozen[b] = frozen ? 1 : 0;
+  bool any = false;
+  for (uint8_t v : c->inst_frozen) any = any || v;
+  if (frozen) {
+    if (!c->mask) {
+      if (!(c->mask = (uint8_t*)dev_alloc<char>(n * c->B, c->bytes))) return PFDD_E_ALLOC;
+      HIPCK(hipMemsetAsync(c->mask, 0, n * c->B, c->stream));
+    }
+    HIPCK(hipMemcpyAsync(c->mask + off, m.data(), n, hipMemcpyHostToDevice, c->stream));
   } else if (c->mask) {
-    (void)hipFree(c->mask);
-    c->mask = nullptr;
+    if (!any) {
+      HIPCK(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->mask);
+      c->mask = nullptr;
+    } else {
+      HIPCK(hipMemsetAsync(c->mask + off, 0, n, c->stream));
+    }
   }
   HIPCK(hipStreamSynchronize(c->stream));
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
-  }
-  c->has_xi = 1;
+  drop_graph(c);
+  c->inst_up[b] = 1;
+  bool all = true;
+  for (uint8_t v : c->inst_up) all = all && v;
+  c->has_xi = all ? 1 : 0;
   c->prologue_done = 0;
   return PFDD_OK;
 }
 
-int pfdd_download_xi(pfdd_ctx* c, double* xi, int nsv) {
-  if (!c || !xi || nsv < c->S) return PFDD_E_SHAPE;
+int download_inst(pfdd_ctx* c, int b, double* xi, int nsv) {
+  if (!xi || nsv < c->S) return PFDD_E_SHAPE;
   const long P = c->P;
+  const size_t n = (size_t)c->S * P;
   HIPCK(hipStreamSynchronize(c->stream));
-  HIPCK(hipMemcpyAsync(xi, c->xi, sizeof(double2) * c->S * P, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(xi, c->xi + (size_t)b * n, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
-  if (nsv > c->S) memset(xi + 2 * (long)c->S * P, 0, sizeof(double) * 2 * (size_t)(nsv - c->S) * P);
+  if (nsv > c->S) memset(xi + 2 * n, 0, sizeof(double) * 2 * (size_t)(nsv - c->S) * P);
   return PFDD_OK;
 }
 
-int pfdd_begin_outer(pfdd_ctx* c, int quirk) {
-  if (!c) return PFDD_E_SHAPE;
+int begin_outer(pfdd_ctx* c, int quirk) {
   if (!c->has_xi) return PFDD_E_STATE;
   const int rc = c->fp64 ? prologue<double>(c, quirk ? 1 : 0) : prologue<float>(c, quirk ? 1 : 0);
   if (rc) return rc;
@@ -807,51 +913,239 @@ int pfdd_begin_outer(pfdd_ctx* c, int quirk) {
   return PFDD_OK;
 }
 
-int pfdd_gl_steps(pfdd_ctx* c, int nsteps, double* gamma, double* gamma_a) {
-  if (!c || nsteps < 0) return PFDD_E_SHAPE;
-  if (!c->prologue_done) return PFDD_E_STATE;
-  const int S = c->S;
-  std::vector<double> h;
+// Runs nsteps GL steps of every context in cs (all on one stream), in chunks
+// of up to HIST_CAP steps; GRAPH_STEPS steps of all of them are captured once
+// into *graph and replayed.  After each chunk, sink(ctx index, first step,
+// steps, history rows) receives the gamma history of each context.
+template <typename Sink>
+int run_steps(pfdd_ctx* const* cs, int nc, hipStream_t st, hipGraphExec_t* graph, int nsteps, bool want_hist,
+              Sink&& sink) {
   int done = 0;
+  std::vector<double> h;
   while (done < nsteps) {
     const int chunk = std::min(HIST_CAP, nsteps - done);
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
+    for (int i = 0; i < nc; i++)
+      hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, cs[i]->step_ctr, 0);
     HIPCK(hipPeekAtLastError());
     int left = chunk;
-    if (left >= GRAPH_STEPS && !c->graph) {
+    if (left >= GRAPH_STEPS && !*graph) {
       hipGraph_t g;
-      HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       int rc = PFDD_OK;
-      for (int s = 0; s < GRAPH_STEPS && rc == PFDD_OK; s++) rc = step(c);
-      hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      for (int s = 0; s < GRAPH_STEPS && rc == PFDD_OK; s++)
+        for (int i = 0; i < nc && rc == PFDD_OK; i++) rc = step(cs[i]);
+      hipError_t ec = hipStreamEndCapture(st, &g);
       if (rc) return rc;
       HIPCK(ec);
-      ec = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      ec = hipGraphInstantiate(graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIPCK(ec);
     }
-    while (left >= GRAPH_STEPS && c->graph) {
-      HIPCK(hipGraphLaunch(c->graph, c->stream));
+    while (left >= GRAPH_STEPS && *graph) {
+      HIPCK(hipGraphLaunch(*graph, st));
       left -= GRAPH_STEPS;
     }
     while (left > 0) {
-      int rc = step(c);
-      if (rc) return rc;
+      for (int i = 0; i < nc; i++) {
+        int rc = step(cs[i]);
+        if (rc) return rc;
+      }
       left--;
     }
-    if (gamma || gamma_a) {
-      h.resize((size_t)chunk * (S + 1));
-      HIPCK(hipMemcpyAsync(h.data(), c->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
-      HIPCK(hipStreamSynchronize(c->stream));
-      for (int s = 0; s < chunk; s++) {
-        if (gamma) gamma[done + s] = h[(size_t)s * (S + 1) + S];
-        if (gamma_a)
-          for (int a = 0; a < S; a++) gamma_a[(size_t)(done + s) * S + a] = h[(size_t)s * (S + 1) + a];
+    if (want_hist) {
+      for (int i = 0; i < nc; i++) {
+        const size_t row = (size_t)cs[i]->B * (cs[i]->S + 1);
+        h.resize((size_t)chunk * row);
+        HIPCK(hipMemcpyAsync(h.data(), cs[i]->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        sink(i, done, chunk, h.data());
       }
     }
     done += chunk;
   }
   return PFDD_OK;
+}
+
+int gl_steps_ctx(pfdd_ctx* c, int nsteps, double* gamma, double* gamma_a) {
+  const int S = c->S;
+  return run_steps(&c, 1, c->stream, &c->graph, nsteps, gamma || gamma_a,
+                   [&](int, int first, int n, const double* h) {
+                     for (int s = 0; s < n; s++) {
+                       if (gamma) gamma[first + s] = h[(size_t)s * (S + 1) + S];
+                       if (gamma_a)
+                         for (int a = 0; a < S; a++) gamma_a[(size_t)(first + s) * S + a] = h[(size_t)s * (S + 1) + a];
+                     }
+                   });
+}
+
+// Candidate plan words of the autotuner: the kernel-fusion choices, and for
+// the fused row and column kernels their largest tile and two smaller ones.
+std::vector<int> plan_candidates(pfdd_ctx* c) {
+  const int base[] = {PFDD_PLAN_AUTO,
+                      PFDD_PLAN_NO_PLANE,
+                      PFDD_PLAN_NO_PLANE | PFDD_PLAN_UNFUSED_ROWS,
+                      PFDD_PLAN_NO_PLANE | PFDD_PLAN_UNFUSED_COLS,
+                      PFDD_PLAN_NO_PLANE | PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS,
+                      PFDD_PLAN_NO_PLANE | PFDD_PLAN_FUSED_COLS,
+                      PFDD_PLAN_PLANE,
+                      PFDD_PLAN_PLANE | PFDD_PLAN_UNFUSED_COLS,
+                      PFDD_PLAN_PLANE | PFDD_PLAN_FUSED_COLS};
+  std::vector<int> out;
+  std::vector<long> seen;
+  const int saved = c->plan;
+  auto add = [&](int plan) {
+    if (apply_plan(c, plan) != PFDD_OK) return;
+    const long sig = ((long)c->row_tile << 32) | ((long)c->c3_tile << 8) | c->plane;
+    for (long v : seen)
+      if (v == sig) return;
+    seen.push_back(sig);
+    out.push_back(plan);
+  };
+  for (int b : base) {
+    add(b);
+    if (apply_plan(c, b) != PFDD_OK) continue;
+    const int rt = c->row_tile, ct = c->c3_tile, pl = c->plane;
+    int lg = 0;
+    if (rt > 0 && !pl)
+      for (lg = 0; (1 << lg) < rt; lg++) {}
+    for (int d = 1; d <= 2 && rt > 0 && !pl && lg - d >= 0; d++) add(b | ((lg - d + 1) << PFDD_PLAN_ROW_TILE_SHIFT));
+    if (ct > 0)
+      for (lg = 0; (1 << lg) < ct; lg++) {}
+    for (int d = 1; d <= 2 && ct > 0 && lg - d >= 0; d++) add(b | ((lg - d + 1) << PFDD_PLAN_COL_TILE_SHIFT));
+  }
+  (void)apply_plan(c, saved);
+  return out;
+}
+
+// Times every candidate plan over `steps` GL steps (after GRAPH_STEPS
+// warm-up steps that also capture the graph) and applies the fastest.  xi and
+// the step counters are saved before and restored after each candidate, so
+// the context's state is as before except that a new prologue is needed.
+int autotune_ctx(pfdd_ctx* c, int steps, int* best_plan, float* best_ms) {
+  if (!c->has_xi) return PFDD_E_STATE;
+  if (steps < 1) return PFDD_E_SHAPE;
+  const size_t xbytes = sizeof(double2) * (size_t)c->F * c->P;
+  void* xsave = nullptr;
+  int* csave = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PFDD_OK;
+  const std::vector<int> cands = plan_candidates(c);
+  int best = c->plan;
+  float best_t = 0.f;
+  if (hipMalloc(&xsave, xbytes) != hipSuccess) return PFDD_E_ALLOC;
+  if (hipMalloc(&csave, sizeof(int) * 4) != hipSuccess) {
+    (void)hipFree(xsave);
+    return PFDD_E_ALLOC;
+  }
+  auto restore = [&]() -> int {
+    HIPCK(hipMemcpyAsync(c->xi, xsave, xbytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->step_ctr, csave, sizeof(int) * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return PFDD_OK;
+  };
+  const int timed = ((steps + GRAPH_STEPS - 1) / GRAPH_STEPS) * GRAPH_STEPS;
+  bool saved = false;
+  do {
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
+    if (hipMemcpyAsync(xsave, c->xi, xbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(csave, c->step_ctr, sizeof(int) * 4, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = PFDD_E_HIP;
+      break;
+    }
+    saved = true;
+    for (int plan : cands) {
+      if ((rc = apply_plan(c, plan))) break;
+      if ((rc = begin_outer(c, 0))) break;
+      if ((rc = gl_steps_ctx(c, GRAPH_STEPS, nullptr, nullptr))) break;
+      if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = PFDD_E_HIP; break; }
+      if ((rc = gl_steps_ctx(c, timed, nullptr, nullptr))) break;
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
+      ms /= (float)timed;
+      if (best_t == 0.f || ms < best_t) {
+        best_t = ms;
+        best = plan;
+      }
+      if ((rc = restore())) break;
+    }
+  } while (0);
+  if (saved) {
+    const int rc1 = restore();
+    if (rc == PFDD_OK) rc = rc1;
+  }
+  const int rc2 = apply_plan(c, best);
+  if (rc == PFDD_OK) rc = rc2;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(xsave);
+  (void)hipFree(csave);
+  if (rc == PFDD_OK) {
+    if (best_plan) *best_plan = best;
+    if (best_ms) *best_ms = best_t;
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfdd_create(const pfdd_params* p, pfdd_ctx** out) {
+  if (!p || !out) return PFDD_E_SHAPE;
+  return ctx_create(p, 1, nullptr, out);
+}
+
+int pfdd_set_plan(pfdd_ctx* c, int plan) {
+  if (!c) return PFDD_E_SHAPE;
+  HIPCK(hipStreamSynchronize(c->stream));
+  return apply_plan(c, plan);
+}
+
+int pfdd_get_plan(const pfdd_ctx* c) { return c ? c->plan : PFDD_E_SHAPE; }
+
+int pfdd_autotune(pfdd_ctx* c, int steps, int* plan, float* ms_per_step) {
+  if (!c) return PFDD_E_SHAPE;
+  return autotune_ctx(c, steps, plan, ms_per_step);
+}
+
+int pfdd_set_tau_ramp(pfdd_ctx* c, const double* tau0, const double* rate) {
+  if (!c || !tau0 || !rate) return PFDD_E_SHAPE;
+  for (int a = 0; a < c->S; a++)
+    if (!std::isfinite(tau0[a]) || !std::isfinite(rate[a])) return PFDD_E_NAN;
+  int rc = pfdd_set_tau(c, tau0, 0);
+  if (rc) return rc;
+  for (int a = 0; a < c->S; a++) c->tauh[a].y = rate[a];
+  return push_tau(c);
+}
+
+int pfdd_set_tau(pfdd_ctx* c, const double* tau, int per_point) {
+  if (!c || !tau) return PFDD_E_SHAPE;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr + 1, 0);  // restart the ramp clock
+  HIPCK(hipPeekAtLastError());
+  return set_tau_inst(c, 0, tau, per_point);
+}
+
+int pfdd_upload_xi(pfdd_ctx* c, const double* xi, const double* xi_bc, int nsv) {
+  if (!c) return PFDD_E_SHAPE;
+  return upload_inst(c, 0, xi, xi_bc, nsv);
+}
+
+int pfdd_download_xi(pfdd_ctx* c, double* xi, int nsv) {
+  if (!c) return PFDD_E_SHAPE;
+  return download_inst(c, 0, xi, nsv);
+}
+
+int pfdd_begin_outer(pfdd_ctx* c, int quirk) {
+  if (!c) return PFDD_E_SHAPE;
+  return begin_outer(c, quirk);
+}
+
+int pfdd_gl_steps(pfdd_ctx* c, int nsteps, double* gamma, double* gamma_a) {
+  if (!c || nsteps < 0) return PFDD_E_SHAPE;
+  if (!c->prologue_done) return PFDD_E_STATE;
+  return gl_steps_ctx(c, nsteps, gamma, gamma_a);
 }
 
 int pfdd_strain_stress(pfdd_ctx* c, const double* avepst_uniform, double* aveps, double* avesigma,
@@ -890,8 +1184,8 @@ namespace {
 template <typename T>
 int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
   cx<T>* W = (cx<T>*)c->W;
-  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, S = c->S;
-  const long n = (long)S * c->P;
+  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
+  const long n = (long)F * c->P;
   int rc = PFDD_OK;
   for (int r = 0; r < reps && rc == PFDD_OK; r++) {
     const int sign = (r & 1) ? PFDD_BACKWARD : PFDD_FORWARD;
@@ -900,18 +1194,19 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
         rc = c->plane ? plane_launch<T>(c, false) : c->row_tile > 0 ? k1_launch<T>(c, false) : update<T>(c, false);
         break;
       case PFDD_STAGE_FINISH:
-        hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->S,
-                           c->plane ? c->p.n1 : c->row_tile > 0 ? c->row_nblk : c->upd_nblk, c->hist, (int*)nullptr);
+        hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F,
+                           c->plane ? c->p.n1 : c->row_tile > 0 ? c->row_nblk : c->upd_nblk, c->S, c->hist,
+                           (int*)nullptr);
         rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
         break;
       case PFDD_STAGE_FFT_AXIS3:
-        rc = launch_axis<T>(W, (long)S * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream);
+        rc = launch_axis<T>(W, (long)F * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream);
         break;
       case PFDD_STAGE_FFT_AXIS2:
-        rc = launch_axis<T>(W, (long)S * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream);
+        rc = launch_axis<T>(W, (long)F * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream);
         break;
       case PFDD_STAGE_FFT_AXIS1:
-        rc = launch_axis<T>(W, (long)S, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream);
+        rc = launch_axis<T>(W, (long)F, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream);
         break;
       case PFDD_STAGE_CONTRACT:
         rc = c->c3_tile > 0 ? k3_launch<T>(c, 0) : contract<T>(c, 0);
@@ -952,6 +1247,199 @@ extern "C" int pfdd_enqueue_stage(pfdd_ctx* c, int stage, int reps) {
   c->prologue_done = 0;
   return c->fp64 ? enqueue_stage<double>(c, stage, reps) : enqueue_stage<float>(c, stage, reps);
 }
+
+// ---------------------------------------------------------------- ensembles
+// Instances with the same grid and material (every pfdd_params field but the
+// applied shear, which is set per instance) form a class; each class is one
+// batched context, and every class runs on the ensemble's stream, so that
+// GRAPH_STEPS steps of the whole ensemble are one captured graph.
+struct pfdd_ens {
+  int count = 0;
+  hipStream_t stream = nullptr;
+  std::vector<pfdd_ctx*> cls;
+  std::vector<int> cls_of, slot_of;  // per instance
+  hipGraphExec_t graph = nullptr;
+  int prologue_done = 0;
+};
+
+namespace {
+
+bool same_class(const pfdd_params& a, const pfdd_params& b) {
+  if (a.n1 != b.n1 || a.n2 != b.n2 || a.n3 != b.n3 || a.ns != b.ns || a.precision != b.precision ||
+      a.bmode != b.bmode || a.plan != b.plan)
+    return false;
+  const double x[] = {a.C11, a.C12, a.C44, a.d1, a.d2, a.d3, a.uq2, a.pi_const};
+  const double y[] = {b.C11, b.C12, b.C44, b.d1, b.d2, b.d3, b.uq2, b.pi_const};
+  for (int i = 0; i < 8; i++)
+    if (x[i] != y[i]) return false;
+  for (int s = 0; s < a.ns; s++) {
+    if (a.cd2[s] != b.cd2[s] || a.asf2[s] != b.asf2[s] || a.dslip[s] != b.dslip[s]) return false;
+    for (int i = 0; i < 9; i++)
+      if (a.eps[s][i / 3][i % 3] != b.eps[s][i / 3][i % 3]) return false;
+  }
+  return true;
+}
+
+void ens_drop_graph(pfdd_ens* e) {
+  if (e->graph) {
+    (void)hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+  }
+}
+
+bool ens_inst_ok(const pfdd_ens* e, int inst) { return e && inst >= 0 && inst < e->count; }
+
+}  // namespace
+
+extern "C" {
+
+void pfdd_destroy_ensemble(pfdd_ens* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  for (pfdd_ctx* c : e->cls) pfdd_destroy(c);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int pfdd_create_ensemble(const pfdd_params* params, int count, pfdd_ens** out) {
+  if (!params || !out || count < 1) return PFDD_E_SHAPE;
+  *out = nullptr;
+  for (int i = 0; i < count; i++) {
+    const int rc = validate_params(&params[i]);
+    if (rc) return rc;
+  }
+  pfdd_ens* e = new (std::nothrow) pfdd_ens();
+  if (!e) return PFDD_E_ALLOC;
+  e->count = count;
+  e->cls_of.assign((size_t)count, -1);
+  e->slot_of.assign((size_t)count, 0);
+  std::vector<int> first;  // first instance of each class
+  std::vector<int> size;
+  for (int i = 0; i < count; i++) {
+    int k = 0;
+    while (k < (int)first.size() && !same_class(params[first[k]], params[i])) k++;
+    if (k == (int)first.size()) {
+      first.push_back(i);
+      size.push_back(0);
+    }
+    e->cls_of[i] = k;
+    e->slot_of[i] = size[k]++;
+  }
+  int rc = PFDD_OK;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    rc = PFDD_E_HIP;
+  } else {
+    for (size_t k = 0; k < first.size() && rc == PFDD_OK; k++) {
+      pfdd_ctx* c = nullptr;
+      rc = ctx_create(&params[first[k]], size[k], e->stream, &c);
+      if (rc == PFDD_OK) e->cls.push_back(c);
+    }
+  }
+  if (rc) {
+    pfdd_destroy_ensemble(e);
+    return rc;
+  }
+  *out = e;
+  return PFDD_OK;
+}
+
+int pfdd_ensemble_info(const pfdd_ens* e, int* nclasses, int* class_of, int* plans) {
+  if (!e) return PFDD_E_SHAPE;
+  if (nclasses) *nclasses = (int)e->cls.size();
+  if (class_of)
+    for (int i = 0; i < e->count; i++) class_of[i] = e->cls_of[i];
+  if (plans)
+    for (size_t k = 0; k < e->cls.size(); k++) plans[k] = e->cls[k]->plan;
+  return PFDD_OK;
+}
+
+int pfdd_ensemble_set_tau(pfdd_ens* e, int inst, const double* tau, int per_point) {
+  if (!ens_inst_ok(e, inst) || !tau) return PFDD_E_SHAPE;
+  ens_drop_graph(e);
+  return set_tau_inst(e->cls[e->cls_of[inst]], e->slot_of[inst], tau, per_point);
+}
+
+int pfdd_ensemble_upload_xi(pfdd_ens* e, int inst, const double* xi, const double* xi_bc, int nsv) {
+  if (!ens_inst_ok(e, inst)) return PFDD_E_SHAPE;
+  ens_drop_graph(e);
+  e->prologue_done = 0;
+  return upload_inst(e->cls[e->cls_of[inst]], e->slot_of[inst], xi, xi_bc, nsv);
+}
+
+int pfdd_ensemble_download_xi(pfdd_ens* e, int inst, double* xi, int nsv) {
+  if (!ens_inst_ok(e, inst)) return PFDD_E_SHAPE;
+  return download_inst(e->cls[e->cls_of[inst]], e->slot_of[inst], xi, nsv);
+}
+
+int pfdd_ensemble_begin_outer(pfdd_ens* e, int quirk) {
+  if (!e) return PFDD_E_SHAPE;
+  for (pfdd_ctx* c : e->cls)
+    if (!c->has_xi) return PFDD_E_STATE;
+  for (pfdd_ctx* c : e->cls) {
+    const int rc = begin_outer(c, quirk);
+    if (rc) return rc;
+  }
+  e->prologue_done = 1;
+  return PFDD_OK;
+}
+
+int pfdd_ensemble_gl_steps(pfdd_ens* e, int nsteps, double* gamma) {
+  if (!e || nsteps < 0) return PFDD_E_SHAPE;
+  if (!e->prologue_done) return PFDD_E_STATE;
+  // instances of each class in slot order
+  std::vector<std::vector<int>> members(e->cls.size());
+  for (int i = 0; i < e->count; i++) members[e->cls_of[i]].push_back(i);
+  const int count = e->count;
+  return run_steps(e->cls.data(), (int)e->cls.size(), e->stream, &e->graph, nsteps, gamma != nullptr,
+                   [&](int k, int first, int n, const double* h) {
+                     const int S = e->cls[k]->S;
+                     const size_t row = (size_t)e->cls[k]->B * (S + 1);
+                     for (int s = 0; s < n; s++)
+                       for (size_t b = 0; b < members[k].size(); b++)
+                         gamma[(size_t)(first + s) * count + members[k][b]] = h[(size_t)s * row + b * (S + 1) + S];
+                   });
+}
+
+int pfdd_ensemble_autotune(pfdd_ens* e, int steps, int* plans, float* ms_per_step) {
+  if (!e) return PFDD_E_SHAPE;
+  ens_drop_graph(e);
+  e->prologue_done = 0;
+  for (size_t k = 0; k < e->cls.size(); k++) {
+    int plan = 0;
+    float ms = 0.f;
+    const int rc = autotune_ctx(e->cls[k], steps, &plan, &ms);
+    if (rc) return rc;
+    if (plans) plans[k] = plan;
+    if (ms_per_step) ms_per_step[k] = ms;
+  }
+  return PFDD_OK;
+}
+
+int pfdd_ensemble_set_plan(pfdd_ens* e, int cls, int plan) {
+  if (!e || cls < 0 || cls >= (int)e->cls.size()) return PFDD_E_SHAPE;
+  ens_drop_graph(e);
+  e->prologue_done = 0;
+  HIPCK(hipStreamSynchronize(e->stream));
+  return apply_plan(e->cls[cls], plan);
+}
+
+int pfdd_ensemble_sync(pfdd_ens* e) {
+  if (!e) return PFDD_E_SHAPE;
+  HIPCK(hipStreamSynchronize(e->stream));
+  return PFDD_OK;
+}
+
+void* pfdd_ensemble_stream(pfdd_ens* e) { return e ? (void*)e->stream : nullptr; }
+
+long long pfdd_ensemble_device_bytes(const pfdd_ens* e) {
+  long long b = 0;
+  if (e)
+    for (const pfdd_ctx* c : e->cls) b += c->bytes;
+  return b;
+}
+
+}  // extern "C"
 
 // ---------------------------------------------------------------- FFT seam on host arrays
 // Process-wide, like the static plans of G.c:124,170; not thread-safe.

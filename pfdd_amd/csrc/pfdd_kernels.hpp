@@ -1,11 +1,13 @@
 // pfdd_kernels.hpp -- HIP kernels of the GL step (gfx950).
 //
-// Data layout in HBM (one context):
-//   xi    double2 [S][N1][N2][N3]   FP64 master order parameter (complex, as G.c:64)
-//   W     cx<T>   [S][N1][N2][N3]   FFT work array: data / data2 of the reference
-//   Bsym  T       [S(S+1)/2][P]     upper triangle of B(k), stored mode only
-//   mask  u8      [S][P]            1 = frozen (xi_bc != 0), only when some point is frozen
-//   taupp double  [P][S]            per-point resolved shear, only when not uniform
+// Data layout in HBM (one context of B instances of the same grid, S slip
+// systems each; field f = b * S + a; B = 1 for a single run):
+//   xi    double2 [B*S][N1][N2][N3]   FP64 master order parameter (complex, as G.c:64)
+//   W     cx<T>   [B*S][N1][N2][N3]   FFT work array: data / data2 of the reference
+//   Bsym  T       [S(S+1)/2][P]       upper triangle of B(k), stored mode only (shared by the instances)
+//   mask  u8      [B*S][P]            1 = frozen (xi_bc != 0), only when some point is frozen
+//   taupp double  [B][P][S]           per-point resolved shear, only when not uniform
+//   tau   double2 [B*S]               uniform resolved shear and its ramp rate per field
 // T = double (PFDD_FP64) or float (PFDD_MIXED).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -170,27 +172,58 @@ struct StepArgs {
   long P;
   int ns;
   int nblk;             // blocks per field of k_update
-  double invP_unused;
   double uq2, pi;
-  double cd2[MAXS], asf2[MAXS], dslip[MAXS], tau[MAXS];
-  double tau_rate[MAXS];   // tau_a(s) = tau_a + tau_rate_a * s, s = GL steps since the ramp was set
+  double cd2[MAXS], asf2[MAXS], dslip[MAXS];  // per slip system
+  // per field f = instance * ns + system: (tau, rate); the s-th GL step
+  // after the ramp was set uses tau + rate * s
+  const double2* tau;
 };
+
+// Per-field coefficients of plasticevolv (G.c:3470-3501) for one block.
+struct FieldCoef {
+  double cd2, asf2, dsl, tau, rate;
+  long tbase;  // per-point resolved shear of point pt: taupp[tbase + pt * ns] ([instance][P][ns])
+};
+__device__ __forceinline__ FieldCoef field_coef(const StepArgs& a, int f, const int* __restrict__ step_ctr) {
+  const int s = f % a.ns;
+  FieldCoef c;
+  c.cd2 = a.cd2[s];
+  c.asf2 = a.asf2[s];
+  c.dsl = a.dslip[s];
+  const double2 t = a.tau[f];
+  c.rate = t.y * (double)step_ctr[1];
+  c.tau = t.x + c.rate;
+  c.tbase = (long)(f / a.ns) * a.P * a.ns + s;
+  return c;
+}
+// The update of one point (G.c:3485-3489): d is the unnormalised inverse
+// transform of data2.
+template <bool TAUPP>
+__device__ __forceinline__ void gl_point(double2& x, double dre, double dim, const FieldCoef& c, const StepArgs& a,
+                                         const double* __restrict__ taupp, long pt, double nsize) {
+  const double t = TAUPP ? taupp[c.tbase + pt * a.ns] + c.rate : c.tau;
+  x.x -= c.cd2 * (a.uq2 * dre / nsize + c.asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / c.dsl);
+  x.y -= a.uq2 * c.cd2 * (dim / nsize);
+}
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 
-// Second pass of the per-field sums: wave f of the single block adds
-// partials[f][0..nblk) in a fixed order (lane-strided, then a butterfly over
-// the wave), so no block-wide barrier sits inside the loop; then thread 0
-// adds the fields in order.  Writes out[f] and out[ns] = sum over fields;
-// with step_ctr the row is out + step_ctr[0](ns+1) and both counters advance
-// (step_ctr[0]: history row, step_ctr[1]: GL steps under the current tau
-// ramp).  ns <= FIN_MAX_FIELDS.
-constexpr int FIN_THREADS = 1024, FIN_MAX_FIELDS = 32;
+// Second pass of the per-field sums: wave w of the single block adds
+// partials[f][0..nblk) for f = w, w + 16, ... in a fixed order (lane-strided,
+// then a butterfly over the wave), so no block-wide barrier sits inside the
+// loop; then each instance's fields (ns consecutive ones) are added in order.
+// Output row: [instance][ns + 1] = the ns field sums and their total.  With
+// step_ctr the row is out + step_ctr[0] * row length and both counters
+// advance (step_ctr[0]: history row, step_ctr[1]: GL steps under the current
+// tau ramp).  nfields <= FIN_MAX_FIELDS, a multiple of ns.
+constexpr int FIN_THREADS = 1024, FIN_MAX_FIELDS = 4096;
 __global__ void __launch_bounds__(FIN_THREADS)
-k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out, int* step_ctr) {
+k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns, double* out, int* step_ctr) {
   __shared__ double fsum[FIN_MAX_FIELDS];
   const int lane = threadIdx.x & 63;
-  for (int f = threadIdx.x >> 6; f < ns; f += FIN_THREADS / 64) {
+  const int ninst = nfields / ns;
+  const long row = step_ctr ? (long)step_ctr[0] : 0;
+  for (int f = threadIdx.x >> 6; f < nfields; f += FIN_THREADS / 64) {
     double v = 0.0;
     for (int b = lane; b < nblk; b += 64) v += partials[(long)f * nblk + b];
 #pragma unroll
@@ -198,24 +231,24 @@ k_finish_sums(const double* __restrict__ partials, int ns, int nblk, double* out
     if (lane == 0) fsum[f] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double* dst = out;
-    if (step_ctr) dst = out + (long)(*step_ctr) * (ns + 1);
+  double* dst = out + row * (long)ninst * (ns + 1);
+  for (int b = threadIdx.x; b < ninst; b += FIN_THREADS) {
     double tot = 0.0;
     for (int g = 0; g < ns; g++) {
-      dst[g] = fsum[g];
-      tot += fsum[g];
+      const double v = fsum[b * ns + g];
+      dst[b * (ns + 1) + g] = v;
+      tot += v;
     }
-    dst[ns] = tot;
-    if (step_ctr) {
-      step_ctr[0] = step_ctr[0] + 1;  // history row
-      step_ctr[1] = step_ctr[1] + 1;  // GL steps since the tau ramp was set
-    }
+    dst[b * (ns + 1) + ns] = tot;
+  }
+  if (step_ctr && threadIdx.x == 0) {
+    step_ctr[0] = step_ctr[0] + 1;  // history row (read by every thread before the barrier above)
+    step_ctr[1] = step_ctr[1] + 1;  // GL steps since the tau ramp was set
   }
 }
 
 // plasticevolv (G.c:3470-3501) fused with the copy xi -> data (G.c:4581-4590)
-// and the gamma sums.  grid = (nblk, S).
+// and the gamma sums.  grid = (nblk, B*S).
 template <typename T, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__(UPD_THREADS)
 k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
@@ -227,9 +260,7 @@ k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restr
   const double nsize = (double)P;
   const long off = (long)f * P;
   const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
-  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
-  const double rate = a.tau_rate[f] * (double)step_ctr[1];
-  const double tauf = a.tau[f] + rate;
+  const FieldCoef c = field_coef(a, f, step_ctr);
   double acc = 0.0;
 #pragma unroll
   for (int e = 0; e < UPD_EPT; e++) {
@@ -238,9 +269,7 @@ k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restr
       double2 x = xi[off + pt];
       const cx<T> d = W[off + pt];
       if (!MASK || mask[off + pt] == 0) {
-        const double t = TAUPP ? taupp[pt * a.ns + f] + rate : tauf;
-        x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / dsl);
-        x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
+        gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, nsize);
         xi[off + pt] = x;
       }
       W[off + pt] = {(T)x.x, (T)x.y};
@@ -392,6 +421,7 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (pt >= P) return;
   const int S = ec->ns;
+  W += (long)blockIdx.y * S * P;  // instance
   cx<T> v[SM];
 #pragma unroll
   for (int b = 0; b < SM; b++)
@@ -427,18 +457,14 @@ k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __res
   const int pen = (int)threadIdx.x / TPP, q = (int)threadIdx.x % TPP;
   fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, tw);
   const double nsize = (double)P;
-  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
-  const double rate = a.tau_rate[f] * (double)step_ctr[1];
-  const double tauf = a.tau[f] + rate;
+  const FieldCoef c = field_coef(a, f, step_ctr);
   double acc = 0.0;
   for (int e = threadIdx.x; e < tile * N; e += NT) {
     const int li = (e / N) * LD + (e % N);
     const cx<T> d = sm[li];
     double2 x = xb[e];
     if (!MASK || mask[off + e] == 0) {
-      const double t = TAUPP ? taupp[(pt0 + e) * a.ns + f] + rate : tauf;
-      x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / dsl);
-      x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
+      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt0 + e, nsize);
       xb[e] = x;
     }
     sm[li] = {(T)x.x, (T)x.y};
@@ -489,9 +515,7 @@ k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __r
   // inverse along N3 (rows)
   fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)1, tw3);
   const double nsize = (double)P;
-  const double cd2 = a.cd2[f], asf2 = a.asf2[f], dsl = a.dslip[f];
-  const double rate = a.tau_rate[f] * (double)step_ctr[1];
-  const double tauf = a.tau[f] + rate;
+  const FieldCoef c = field_coef(a, f, step_ctr);
   double2* xb = xi + off;
   double acc = 0.0;
   for (int e = t; e < PLANE; e += PLANE_THREADS) {
@@ -499,9 +523,7 @@ k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __r
     const cx<T> d = sm[li];
     double2 x = xb[e];
     if (!MASK || mask[off + e] == 0) {
-      const double tt = TAUPP ? taupp[((long)i1 * PLANE + e) * a.ns + f] + rate : tauf;
-      x.x -= cd2 * (a.uq2 * (double)d.x / nsize + asf2 * a.pi * sin(2.0 * a.pi * x.x) - tt / dsl);
-      x.y -= a.uq2 * cd2 * ((double)d.y / nsize);
+      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, (long)i1 * PLANE + e, nsize);
       xb[e] = x;
     }
     sm[li] = {(T)x.x, (T)x.y};
@@ -535,6 +557,7 @@ k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   extern __shared__ unsigned char lds_raw[];
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   const int S = ec->ns;
+  W += (long)blockIdx.y * S * P;  // instance
   const long inner = P / N;
   const long m0 = (long)blockIdx.x * tile;
   const int npen = S * tile;

@@ -3,6 +3,7 @@ mixed-precision error against the FP64 run of the same instance.
 
     python -m pfdd_amd.run_config c2 [--steps N] [--json out.json]
     python -m pfdd_amd.run_config c4 [--steps N] [--every 50] [--json out.json]
+    python -m pfdd_amd.run_config c3 [--steps N] [--count 512] [--json out.json]
 
 c2: 256^3 x 12, 64 seeded loops: mixed and FP64 GL runs of N steps (default
     2000); reports GL steps/s, grid-points x systems / s, and the errors of the
@@ -19,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import B_RECOMPUTE, B_STORED, FP64, MIXED, Context
+from . import B_RECOMPUTE, B_STORED, FP64, MIXED, Context, Ensemble, describe_plan
 from . import configs as C
 from . import hip
 
@@ -79,6 +80,55 @@ def run_ramp(w, steps, precision, bmode, every):
     return out, xi, curve
 
 
+def _timed_ens(e, n):
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(e.stream)
+    e.gl_steps(n, record=False)
+    e1.record(e.stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1)
+
+
+def build_ensemble(w, precision):
+    e = Ensemble(w.params_list(precision, B_RECOMPUTE))
+    for i in range(w.count):
+        m = w.member(i)
+        e.set_tau(i, m.tau)
+        e.upload_xi(i, m.xi)
+    return e
+
+
+def run_ensemble(w, steps, precision, tune_steps=8):
+    """AUTO-plan timing, then per-class autotuning and the timed run."""
+    t0 = time.time()
+    e = build_ensemble(w, precision)
+    setup_s = time.time() - t0
+    ncls, cls, _ = e.info()
+    e.begin_outer(True)
+    e.gl_steps(8, record=False)
+    auto_ms = _timed_ens(e, 64) / 64
+    # back to the initial state for the measured run
+    for i in range(w.count):
+        e.upload_xi(i, w.member(i).xi)
+    t1 = time.time()
+    plans, tuned_ms = e.autotune(tune_steps)
+    tune_s = time.time() - t1
+    e.begin_outer(True)
+    ms = _timed_ens(e, steps)
+    classes = []
+    for k in range(ncls):
+        members = np.nonzero(cls == k)[0]
+        classes.append({"edge": w.shape(int(members[0]))[0], "instances": int(len(members)),
+                        "plan": int(plans[k]), "plan_name": describe_plan(int(plans[k])),
+                        "ms_per_step": float(tuned_ms[k])})
+    pts = w.total_points() * w.ns
+    out = {"steps": steps, "ms": ms, "steps_per_s": steps / (ms / 1e3),
+           "gridpoints_x_systems_per_s": pts * steps / (ms / 1e3),
+           "auto_plan_ms_per_step": auto_ms, "auto_plan_gridpoints_x_systems_per_s": pts / (auto_ms / 1e3),
+           "classes": classes, "setup_s": setup_s, "autotune_s": tune_s, "device_bytes": e.device_bytes}
+    return out, e
+
+
 def step_bytes(S, P, fp64, stored):
     if fp64:
         return (144 * S + (4 * S * (S + 1) if stored else 0)) * P
@@ -87,7 +137,8 @@ def step_bytes(S, P, fp64, stored):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["c2", "c4"])
+    ap.add_argument("config", choices=["c2", "c3", "c4"])
+    ap.add_argument("--count", type=int, default=512)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--every", type=int, default=50)
     ap.add_argument("--json", default=None)
@@ -112,6 +163,30 @@ def main():
                     "mean_stress_fp64": s64.tolist(),
                     "step_roofline_frac_mixed": step_bytes(w.ns, w.P, False, True) * r32["steps_per_s"] / 8e12,
                     "step_roofline_frac_fp64": step_bytes(w.ns, w.P, True, False) * r64["steps_per_s"] / 8e12})
+    elif a.config == "c3":
+        w = C.c3(count=a.count)
+        steps = a.steps or w.steps
+        print(f"[run_config] c3 {w.count} instances, {w.total_points()} points x 3, {steps} steps", flush=True)
+        r32, e32 = run_ensemble(w, steps, MIXED)
+        print(f"[run_config] mixed {r32['steps_per_s']:.2f} ensemble steps/s "
+              f"({r32['gridpoints_x_systems_per_s']:.3e} pts*sys/s)", flush=True)
+        r64, e64 = run_ensemble(w, steps, FP64)
+        print(f"[run_config] fp64 {r64['steps_per_s']:.2f} ensemble steps/s", flush=True)
+        xerr, aerr = 0.0, 0.0
+        for i in range(w.count):
+            x32, x64 = e32.download_xi(i), e64.download_xi(i)
+            xerr = max(xerr, float(np.abs(x32 - x64).max()))
+            a32, a64 = float(np.sum(x32.real)), float(np.sum(x64.real))   # slipped area, sum_a gamma_a P
+            aerr = max(aerr, abs(a32 - a64) / max(abs(a64), 1e-300))
+        e32.close()
+        e64.close()
+        # algorithmic bytes of one ensemble step (recompute B): 96 S P mixed, 144 S P FP64
+        pts = w.total_points()
+        res.update({"count": w.count, "ns": w.ns, "total_points": pts, "steps": steps,
+                    "mixed": r32, "fp64": r64, "speedup": r32["steps_per_s"] / r64["steps_per_s"],
+                    "xi_max_abs_err": xerr, "slipped_area_max_rel_err": aerr,
+                    "step_roofline_frac_mixed": 96 * 3 * pts * r32["steps_per_s"] / 8e12,
+                    "step_roofline_frac_fp64": 144 * 3 * pts * r64["steps_per_s"] / 8e12})
     else:
         w = C.c4(steps=a.steps or 5000)
         steps = w.steps
