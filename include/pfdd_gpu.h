@@ -66,7 +66,8 @@ typedef struct {
  * 128^3 x 12; pfdd_autotune measures the candidates on the actual problem.
  * Bits 0-4 are flags; bits 8-11 and 12-15 optionally fix the tile of the
  * fused row / column kernel as log2(tile) + 1 (0 = the largest that fits);
- * a tile that does not fit is rejected with PFDD_E_SHAPE. */
+ * a tile that does not fit is rejected with PFDD_E_SHAPE; bits 16-18 pick
+ * the radix sequence of the standalone passes per axis. */
 enum {
   PFDD_PLAN_AUTO = 0,
   PFDD_PLAN_UNFUSED_ROWS = 1,  /* plain N3 passes + pointwise update instead of the fused row kernel */
@@ -75,7 +76,12 @@ enum {
   PFDD_PLAN_NO_PLANE = 8,      /* no LDS-resident N2 x N3 plane kernel (AUTO uses it for N2 = N3 <= 96) */
   PFDD_PLAN_PLANE = 16,        /* the plane kernel also at 128^2 (mixed precision only) */
   PFDD_PLAN_ROW_TILE_SHIFT = 8,  /* rows along N3 per block of the fused row kernel */
-  PFDD_PLAN_COL_TILE_SHIFT = 12  /* (i2,i3) columns per block of the fused column kernel */
+  PFDD_PLAN_COL_TILE_SHIFT = 12, /* (i2,i3) columns per block of the fused column kernel */
+  /* the standalone FFT passes along an axis take radix-4 butterflies instead
+   * of radix-8 first (more passes, twice the threads per pencil) */
+  PFDD_PLAN_RADIX4_AXIS1 = 1 << 16,
+  PFDD_PLAN_RADIX4_AXIS2 = 1 << 17,
+  PFDD_PLAN_RADIX4_AXIS3 = 1 << 18
 };
 #define PFDD_PLAN_ROW_TILE_LOG2(l) (((l) + 1) << PFDD_PLAN_ROW_TILE_SHIFT)
 #define PFDD_PLAN_COL_TILE_LOG2(l) (((l) + 1) << PFDD_PLAN_COL_TILE_SHIFT)
@@ -145,7 +151,8 @@ int pfdd_set_plan(pfdd_ctx *c, int plan);
 int pfdd_get_plan(const pfdd_ctx *c);
 
 /* Autotuner (no reference counterpart): times every candidate plan (fusion
- * choices x tiles of the fused kernels) over `steps` GL steps of the loaded
+ * choices x tiles of the fused kernels, then radix sequences of the
+ * standalone passes of the best one) over `steps` GL steps of the loaded
  * problem and applies the fastest; *plan and *ms_per_step (may be NULL)
  * receive it and its time.  Needs xi uploaded.  xi, tau and the ramp clock
  * are restored afterwards; pfdd_begin_outer must run again.  Synchronous. */
@@ -175,6 +182,15 @@ enum {
   PFDD_STAGE_FINISH = 7     /* the one-block pass that turns block sums into gamma */
 };
 int pfdd_enqueue_stage(pfdd_ctx *c, int stage, int reps);
+
+/* Profiling aid, no reference counterpart: runs nsteps eager GL steps (as
+ * pfdd_gl_steps, advancing the state) with a HIP event recorded after every
+ * kernel launch, and returns per stage (index PFDD_STAGE_*) the time per
+ * step (ms[8], the sum over that stage's launches in one step) and the
+ * launches per step (launches[8]).  Each launch is timed inside the real
+ * sequence of a step, between the events of its neighbours.  Needs
+ * pfdd_begin_outer.  Synchronous. */
+int pfdd_time_step_kernels(pfdd_ctx *c, int nsteps, float *ms, int *launches);
 
 /* Launches of each stage per GL step for this context (launches[8]):
  * PFDD_STAGE_UPDATE and PFDD_STAGE_CONTRACT name the fused row and column

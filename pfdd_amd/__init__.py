@@ -20,6 +20,7 @@ B_STORED, B_RECOMPUTE = 0, 1
 FORWARD, BACKWARD = -1, 1
 PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS, PLAN_NO_PLANE, PLAN_PLANE = 0, 1, 2, 4, 8, 16
 PLAN_ROW_TILE_SHIFT, PLAN_COL_TILE_SHIFT = 8, 12
+PLAN_RADIX4_AXIS1, PLAN_RADIX4_AXIS2, PLAN_RADIX4_AXIS3 = 1 << 16, 1 << 17, 1 << 18
 
 
 def plan_row_tile(tile):
@@ -34,7 +35,8 @@ def plan_col_tile(tile):
 
 def describe_plan(plan):
     names = [n for n, v in (("unfused_rows", 1), ("unfused_cols", 2), ("fused_cols", 4), ("no_plane", 8),
-                            ("plane", 16)) if plan & v]
+                            ("plane", 16), ("radix4_n1", 1 << 16), ("radix4_n2", 1 << 17),
+                            ("radix4_n3", 1 << 18)) if plan & v]
     rt, ct = (plan >> PLAN_ROW_TILE_SHIFT) & 15, (plan >> PLAN_COL_TILE_SHIFT) & 15
     if rt:
         names.append(f"row_tile={1 << (rt - 1)}")
@@ -53,7 +55,7 @@ EXPORTED = [
     "pfdd_autotune", "pfdd_create_ensemble", "pfdd_destroy_ensemble", "pfdd_ensemble_info",
     "pfdd_ensemble_set_tau", "pfdd_ensemble_upload_xi", "pfdd_ensemble_download_xi", "pfdd_ensemble_begin_outer",
     "pfdd_ensemble_gl_steps", "pfdd_ensemble_autotune", "pfdd_ensemble_set_plan", "pfdd_ensemble_sync",
-    "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes",
+    "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes", "pfdd_time_step_kernels",
 ]
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
  STAGE_FINISH) = range(8)
@@ -137,6 +139,7 @@ def load():
     L.pfdd_fft_host_release.argtypes = []
     L.pfdd_fft_host_release.restype = None
     fp = ctypes.POINTER(ctypes.c_float)
+    L.pfdd_time_step_kernels.argtypes = [vp, ci, vp, vp]
     L.pfdd_set_plan.argtypes = [vp, ci]
     L.pfdd_get_plan.argtypes = [vp]
     L.pfdd_autotune.argtypes = [vp, ci, vp, fp]
@@ -279,6 +282,15 @@ class Context:
         a = (ctypes.c_int * 8)()
         _check(load().pfdd_step_plan(self.h, a), "pfdd_step_plan")
         return list(a)
+
+    def time_step_kernels(self, nsteps=10):
+        """Per stage: (ms per step, launches per step) from events between the
+        launches of nsteps eager GL steps (the state advances by nsteps)."""
+        ms = np.zeros(8, dtype=np.float32)
+        n = np.zeros(8, dtype=np.int32)
+        _check(load().pfdd_time_step_kernels(self.h, nsteps, ms.ctypes.data, n.ctypes.data),
+               "pfdd_time_step_kernels")
+        return {st: (float(ms[st]), int(n[st])) for st in range(8) if n[st] > 0}
 
     def set_plan(self, plan):
         _check(load().pfdd_set_plan(self.h, plan), "pfdd_set_plan")

@@ -79,10 +79,15 @@ struct pfdd_ctx {
   int plane = 0;             // 1: the LDS-resident plane kernel does the N2/N3 passes and the update
   size_t c3_lds = 0;         // its dynamic LDS bytes
   int plan = 0;              // the applied PFDD_PLAN_* word
+  int r4 = 0;                // bit ax: radix-4 sequence for the standalone passes along axis ax
   int has_xi = 0;
   int prologue_done = 0;
   hipGraphExec_t graph = nullptr;
   long long bytes = 0;
+  // kernel timing: when set, an event is recorded after every launch of a
+  // step, tagged with its stage (pfdd_time_step_kernels)
+  std::vector<std::pair<int, hipEvent_t>>* tap = nullptr;
+  std::vector<hipEvent_t>* tap_pool = nullptr;
   // per instance
   std::vector<double2> tauh; // host copy of taud
   std::vector<uint8_t> inst_pp, inst_frozen, inst_up;
@@ -98,32 +103,56 @@ bool len_ok(int n) {
   return false;
 }
 
+// Records the timing event of a launch of `stage` (see pfdd_ctx::tap).
+void tap(pfdd_ctx* c, int stage) {
+  if (!c->tap || c->tap->size() >= c->tap_pool->size()) return;
+  hipEvent_t e = (*c->tap_pool)[c->tap->size()];
+  (void)hipEventRecord(e, c->stream);
+  c->tap->push_back({stage, e});
+}
+
 template <typename T> const cx<T>* tw_of(const Twiddles& t);
 template <> const double2* tw_of<double>(const Twiddles& t) { return t.d; }
 template <> const float2* tw_of<float>(const Twiddles& t) { return t.f; }
 
-template <typename T, int N>
+// radix limit of the standalone passes along axis ax (0: N1, 1: N2, 2: N3)
+int rl_of(const pfdd_ctx* c, int ax) { return (c->r4 >> ax) & 1 ? 4 : 8; }
+
+template <typename T, int N, int RL>
 void launch_axis_n(cx<T>* data, long outer, long inner, int sign, const cx<T>* tw, hipStream_t st) {
-  constexpr int TPP = make_plan(N).tpp;
+  constexpr int TPP = make_plan(N, RL).tpp;
   if (inner == 1) {
-    constexpr int TILE = tile_rows<T, N>();
+    constexpr int TILE = tile_rows<T, N, RL>();
     const long nrows = outer;
     const long nb = (nrows + TILE - 1) / TILE;
-    hipLaunchKernelGGL((k_fft_rows<T, N>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, nrows, (T)sign, tw);
+    hipLaunchKernelGGL((k_fft_rows<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, nrows, (T)sign, tw);
   } else {
-    constexpr int TILE = tile_cols<T, N>();
+    constexpr int TILE = tile_cols<T, N, RL>();
     int tile = 1;
     while (tile * 2 <= TILE && inner % (tile * 2) == 0) tile *= 2;
     const long nb = outer * (inner / tile);
-    hipLaunchKernelGGL((k_fft_cols<T, N>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, inner, tile, (T)sign, tw);
+    hipLaunchKernelGGL((k_fft_cols<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, inner, tile,
+                       (T)sign, tw);
   }
+}
+
+// rl = 4 takes the radix-4 sequence where the length has one
+template <typename T, int N>
+void launch_axis_r(cx<T>* data, long outer, long inner, int sign, const cx<T>* tw, hipStream_t st, int rl) {
+  if constexpr (has_r4_variant(N)) {
+    if (rl == 4) {
+      launch_axis_n<T, N, 4>(data, outer, inner, sign, tw, st);
+      return;
+    }
+  }
+  launch_axis_n<T, N, 8>(data, outer, inner, sign, tw, st);
 }
 
 // Transform along one axis of an [outer][n][inner] complex array.
 template <typename T>
-int launch_axis(cx<T>* data, long outer, int n, long inner, int sign, const cx<T>* tw, hipStream_t st) {
+int launch_axis(cx<T>* data, long outer, int n, long inner, int sign, const cx<T>* tw, hipStream_t st, int rl = 8) {
   switch (n) {
-#define PFDD_LEN(NN) case NN: launch_axis_n<T, NN>(data, outer, inner, sign, tw, st); break;
+#define PFDD_LEN(NN) case NN: launch_axis_r<T, NN>(data, outer, inner, sign, tw, st, rl); break;
     PFDD_LEN(2) PFDD_LEN(3) PFDD_LEN(4) PFDD_LEN(6) PFDD_LEN(8) PFDD_LEN(12) PFDD_LEN(16) PFDD_LEN(24)
     PFDD_LEN(32) PFDD_LEN(48) PFDD_LEN(64) PFDD_LEN(96) PFDD_LEN(128) PFDD_LEN(192) PFDD_LEN(256)
     PFDD_LEN(384) PFDD_LEN(512)
@@ -137,9 +166,12 @@ template <typename T>
 int fft3d(pfdd_ctx* c, cx<T>* data, int batch, int sign) {
   const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3;
   int rc;
-  if ((rc = launch_axis<T>(data, (long)batch * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream))) return rc;
-  if ((rc = launch_axis<T>(data, (long)batch * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream))) return rc;
-  if ((rc = launch_axis<T>(data, (long)batch, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream))) return rc;
+  if ((rc = launch_axis<T>(data, (long)batch * n1 * n2, n3, 1, sign,
+                           tw_of<T>(c->tw[2]), c->stream, rl_of(c, 2)))) return rc;
+  if ((rc = launch_axis<T>(data, (long)batch * n1, n2, n3, sign,
+                           tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1)))) return rc;
+  if ((rc = launch_axis<T>(data, (long)batch, n1, (long)n2 * n3, sign,
+                           tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0)))) return rc;
   return PFDD_OK;
 }
 
@@ -155,6 +187,7 @@ int contract(pfdd_ctx* c, int quirk) {
     hipLaunchKernelGGL((k_contract<T, SM, false>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
   if (c->S <= 4) { PFDD_CT(4) } else if (c->S <= 8) { PFDD_CT(8) } else if (c->S <= 12) { PFDD_CT(12) } else { PFDD_CT(16) }
 #undef PFDD_CT
+  tap(c, PFDD_STAGE_CONTRACT);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -171,9 +204,12 @@ int update(pfdd_ctx* c, bool finish = true) {
   else if (tp) PFDD_UP(false, true);
   else PFDD_UP(false, false);
 #undef PFDD_UP
-  if (finish)
+  tap(c, PFDD_STAGE_UPDATE);
+  if (finish) {
     hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->upd_nblk, c->S,
                        c->hist, c->step_ctr);
+    tap(c, PFDD_STAGE_FINISH);
+  }
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -225,9 +261,12 @@ int k1_launch(pfdd_ctx* c, bool finish = true) {
 #undef PFDD_L
     default: return PFDD_E_RADIX;
   }
-  if (finish)
+  tap(c, PFDD_STAGE_UPDATE);
+  if (finish) {
     hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->row_nblk, c->S,
                        c->hist, c->step_ctr);
+    tap(c, PFDD_STAGE_FINISH);
+  }
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -250,6 +289,7 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
   }
   if (c->S <= 4) { PFDD_K3(4) } else if (c->S <= 8) { PFDD_K3(8) } else if (c->S <= 12) { PFDD_K3(12) } else { PFDD_K3(16) }
 #undef PFDD_K3
+  if (!attr_only) tap(c, PFDD_STAGE_CONTRACT);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -281,7 +321,7 @@ int plane_launch_n(pfdd_ctx* c, bool attr_only) {
     HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_step<T, N, N, MM, TT>),                     \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                           \
   } else {                                                                                                       \
-    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT>), dim3(nb), dim3(PLANE_THREADS), lds, c->stream, W, c->xi,  \
+    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT>), dim3(nb), dim3(plane_threads<T, N, N>()), lds, c->stream, W, c->xi,  \
                        c->mask, c->taupp, c->sa, tw2, tw3, c->partials, c->step_ctr);                           \
   }
   if (attr_only) {
@@ -314,9 +354,12 @@ int plane_launch(pfdd_ctx* c, bool finish = true, bool attr_only = false) {
       return PFDD_E_RADIX;
   }
   if (rc || attr_only) return rc;
-  if (finish)
+  tap(c, PFDD_STAGE_UPDATE);
+  if (finish) {
     hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->p.n1, c->S,
                        c->hist, c->step_ctr);
+    tap(c, PFDD_STAGE_FINISH);
+  }
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -328,9 +371,13 @@ int n1_part(pfdd_ctx* c, int quirk) {
   cx<T>* W = (cx<T>*)c->W;
   const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
   if (c->c3_tile > 0) return k3_launch<T>(c, quirk);
-  if ((rc = launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_FORWARD, tw_of<T>(c->tw[0]), c->stream))) return rc;
+  if ((rc = launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_FORWARD,
+                           tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0)))) return rc;
+  tap(c, PFDD_STAGE_FFT_AXIS1);
   if ((rc = contract<T>(c, quirk))) return rc;
-  return launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream);
+  rc = launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0));
+  tap(c, PFDD_STAGE_FFT_AXIS1);
+  return rc;
 }
 
 // Passes between the forward N3 pass and the inverse N3 pass: forward N2,
@@ -341,15 +388,21 @@ int middle(pfdd_ctx* c, int quirk) {
   int rc;
   cx<T>* W = (cx<T>*)c->W;
   const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
-  if ((rc = launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_FORWARD, tw_of<T>(c->tw[1]), c->stream))) return rc;
+  if ((rc = launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_FORWARD,
+                           tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1)))) return rc;
+  tap(c, PFDD_STAGE_FFT_AXIS2);
   if ((rc = n1_part<T>(c, quirk))) return rc;
-  return launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream);
+  rc = launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1));
+  tap(c, PFDD_STAGE_FFT_AXIS2);
+  return rc;
 }
 
 template <typename T>
 int axis3(pfdd_ctx* c, int sign) {
-  return launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1 * c->p.n2, c->p.n3, 1, sign, tw_of<T>(c->tw[2]),
-                        c->stream);
+  const int rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1 * c->p.n2, c->p.n3, 1, sign,
+                           tw_of<T>(c->tw[2]), c->stream, rl_of(c, 2));
+  tap(c, PFDD_STAGE_FFT_AXIS3);
+  return rc;
 }
 
 // Between GL steps W holds data2 with the inverse transform done along N1
@@ -380,8 +433,8 @@ int prologue(pfdd_ctx* c, int quirk) {
   hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, (cx<T>*)c->W, n);
   if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
   if (c->plane) {  // the next step's plane kernel finishes the N2 and N3 inverse passes
-    if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD, tw_of<T>(c->tw[1]),
-                             c->stream)))
+    if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD,
+                             tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1))))
       return rc;
     return n1_part<T>(c, quirk);
   }
@@ -548,7 +601,8 @@ namespace {
 
 constexpr int PLAN_FLAGS = PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS |
                            PFDD_PLAN_NO_PLANE | PFDD_PLAN_PLANE;
-constexpr int PLAN_TILES = (15 << PFDD_PLAN_ROW_TILE_SHIFT) | (15 << PFDD_PLAN_COL_TILE_SHIFT);
+constexpr int PLAN_TILES = (15 << PFDD_PLAN_ROW_TILE_SHIFT) | (15 << PFDD_PLAN_COL_TILE_SHIFT) |
+                           PFDD_PLAN_RADIX4_AXIS1 | PFDD_PLAN_RADIX4_AXIS2 | PFDD_PLAN_RADIX4_AXIS3;
 
 void drop_graph(pfdd_ctx* c) {
   if (c->graph) {
@@ -611,6 +665,8 @@ int apply_plan(pfdd_ctx* c, int plan) {
   if (!(plan & PFDD_PLAN_NO_PLANE) && p.n2 == p.n3 && plane_len(p.n3, c->fp64) &&
       (p.n3 <= 96 || (plan & PFDD_PLAN_PLANE)))
     plane = 1;
+  c->r4 = ((plan & PFDD_PLAN_RADIX4_AXIS1) ? 1 : 0) | ((plan & PFDD_PLAN_RADIX4_AXIS2) ? 2 : 0) |
+          ((plan & PFDD_PLAN_RADIX4_AXIS3) ? 4 : 0);
   c->row_tile = row_tile;
   c->row_nblk = row_nblk;
   c->c3_tile = c3_tile;
@@ -735,6 +791,8 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
         double se = 0.0;
         for (int mn = 0; mn < 9; mn++) se += h.sig[a][mn] * h.eps[b][mn];
         h.SE[a][b] = se;
+        h.SEm[a][b] = se * h.inv_mu;
+        h.SEmf[a][b] = (float)h.SEm[a][b];
       }
     h.d[0] = p->d1; h.d[1] = p->d2; h.d[2] = p->d3;
     // frec (G.c:2698-2741) per axis, evaluated with the same arithmetic as the reference
@@ -1054,21 +1112,48 @@ int autotune_ctx(pfdd_ctx* c, int steps, int* best_plan, float* best_ms) {
       break;
     }
     saved = true;
-    for (int plan : cands) {
-      if ((rc = apply_plan(c, plan))) break;
-      if ((rc = begin_outer(c, 0))) break;
-      if ((rc = gl_steps_ctx(c, GRAPH_STEPS, nullptr, nullptr))) break;
-      if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = PFDD_E_HIP; break; }
-      if ((rc = gl_steps_ctx(c, timed, nullptr, nullptr))) break;
-      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
-      ms /= (float)timed;
-      if (best_t == 0.f || ms < best_t) {
-        best_t = ms;
-        best = plan;
-      }
-      if ((rc = restore())) break;
+    // Two rounds over a candidate list; each candidate keeps its faster
+    // round (the first also absorbs clock ramp-up and first-launch costs).
+    auto tune = [&](const std::vector<int>& list) -> int {
+      std::vector<float> tmin(list.size(), 0.f);
+      for (int round = 0; round < 2; round++)
+        for (size_t ci = 0; ci < list.size(); ci++) {
+          int r;
+          if ((r = apply_plan(c, list[ci]))) return r;
+          if ((r = begin_outer(c, 0))) return r;
+          if ((r = gl_steps_ctx(c, GRAPH_STEPS, nullptr, nullptr))) return r;
+          HIPCK(hipEventRecord(e0, c->stream));
+          if ((r = gl_steps_ctx(c, timed, nullptr, nullptr))) return r;
+          HIPCK(hipEventRecord(e1, c->stream));
+          HIPCK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCK(hipEventElapsedTime(&ms, e0, e1));
+          ms /= (float)timed;
+          if (tmin[ci] == 0.f || ms < tmin[ci]) tmin[ci] = ms;
+          if ((r = restore())) return r;
+        }
+      best_t = 0.f;
+      for (size_t ci = 0; ci < list.size(); ci++)
+        if (best_t == 0.f || tmin[ci] < best_t) {
+          best_t = tmin[ci];
+          best = list[ci];
+        }
+      return PFDD_OK;
+    };
+    // stage 1: kernel fusion and tiles
+    if ((rc = tune(cands))) break;
+    // stage 2: radix-4 sequences for the standalone passes the best plan uses
+    if ((rc = apply_plan(c, best))) break;
+    const int n3[3] = {c->p.n1, c->p.n2, c->p.n3};
+    const bool standalone[3] = {c->c3_tile == 0, !c->plane, !c->plane && c->row_tile == 0};
+    const int bit[3] = {PFDD_PLAN_RADIX4_AXIS1, PFDD_PLAN_RADIX4_AXIS2, PFDD_PLAN_RADIX4_AXIS3};
+    int axes = 0;
+    for (int ax = 0; ax < 3; ax++)
+      if (standalone[ax] && has_r4_variant(n3[ax])) axes |= bit[ax];
+    if (axes) {
+      std::vector<int> radix = {best};
+      for (int sub = axes; sub > 0; sub = (sub - 1) & axes) radix.push_back(best | sub);
+      if ((rc = tune(radix))) break;
     }
   } while (0);
   if (saved) {
@@ -1200,13 +1285,13 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
         rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
         break;
       case PFDD_STAGE_FFT_AXIS3:
-        rc = launch_axis<T>(W, (long)F * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream);
+        rc = launch_axis<T>(W, (long)F * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream, rl_of(c, 2));
         break;
       case PFDD_STAGE_FFT_AXIS2:
-        rc = launch_axis<T>(W, (long)F * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream);
+        rc = launch_axis<T>(W, (long)F * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1));
         break;
       case PFDD_STAGE_FFT_AXIS1:
-        rc = launch_axis<T>(W, (long)F, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream);
+        rc = launch_axis<T>(W, (long)F, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0));
         break;
       case PFDD_STAGE_CONTRACT:
         rc = c->c3_tile > 0 ? k3_launch<T>(c, 0) : contract<T>(c, 0);
@@ -1240,6 +1325,50 @@ extern "C" int pfdd_step_plan(const pfdd_ctx* c, int* launches) {
   launches[PFDD_STAGE_COPY] = 0;
   launches[PFDD_STAGE_FINISH] = 1;
   return PFDD_OK;
+}
+
+extern "C" int pfdd_time_step_kernels(pfdd_ctx* c, int nsteps, float* ms, int* launches) {
+  if (!c || nsteps < 1 || nsteps > HIST_CAP || !ms || !launches) return PFDD_E_SHAPE;
+  if (!c->prologue_done) return PFDD_E_STATE;
+  const int per_step_max = 16;
+  std::vector<hipEvent_t> pool((size_t)nsteps * per_step_max + 1, nullptr);
+  std::vector<std::pair<int, hipEvent_t>> taps;
+  int rc = PFDD_OK;
+  for (auto& e : pool)
+    if (hipEventCreate(&e) != hipSuccess) rc = PFDD_E_HIP;
+  if (rc == PFDD_OK) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
+    taps.reserve(pool.size());
+    c->tap_pool = &pool;
+    c->tap = &taps;
+    tap(c, -1);  // start mark
+    for (int s = 0; s < nsteps && rc == PFDD_OK; s++) rc = step(c);
+    c->tap = nullptr;
+    c->tap_pool = nullptr;
+    if (rc == PFDD_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = PFDD_E_HIP;
+  }
+  if (rc == PFDD_OK) {
+    for (int k = 0; k < 8; k++) {
+      ms[k] = 0.f;
+      launches[k] = 0;
+    }
+    for (size_t i = 1; i < taps.size(); i++) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, taps[i - 1].second, taps[i].second) != hipSuccess) {
+        rc = PFDD_E_HIP;
+        break;
+      }
+      const int st = taps[i].first;
+      if (st >= 0 && st < 8) {
+        ms[st] += t / (float)nsteps;
+        launches[st]++;
+      }
+    }
+    for (int k = 0; k < 8; k++) launches[k] /= nsteps;
+  }
+  for (auto& e : pool)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
 }
 
 extern "C" int pfdd_enqueue_stage(pfdd_ctx* c, int stage, int reps) {

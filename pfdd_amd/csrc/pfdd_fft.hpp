@@ -36,20 +36,31 @@ struct Plan {
   int tpp;     // threads per pencil = N / max radix
 };
 
-constexpr Plan make_plan(int n) {
+// Radix sequence of an n-point transform: with rlim = 8, radix-8 passes
+// first, then at most one radix-4 and one radix-2 pass, then radix-3 passes;
+// with rlim = 4 the power of two is taken in radix-4 passes (and at most
+// one radix-2 pass): more passes, but twice the threads per pencil and half
+// the registers per thread.
+constexpr Plan make_plan(int n, int rlim = 8) {
   Plan p{};
   int m = n, rmax = 1;
-  while (m % 8 == 0) { p.r[p.np++] = 8; m /= 8; }
-  if (m % 4 == 0) { p.r[p.np++] = 4; m /= 4; }
+  while (rlim >= 8 && m % 8 == 0) { p.r[p.np++] = 8; m /= 8; }
+  if (rlim >= 8) {
+    if (m % 4 == 0) { p.r[p.np++] = 4; m /= 4; }
+  } else {
+    while (m % 4 == 0) { p.r[p.np++] = 4; m /= 4; }
+  }
   if (m % 2 == 0) { p.r[p.np++] = 2; m /= 2; }
   while (m % 3 == 0) { p.r[p.np++] = 3; m /= 3; }
   for (int i = 0; i < p.np; i++) rmax = p.r[i] > rmax ? p.r[i] : rmax;
   p.tpp = n / rmax;
   return p;
 }
+// whether the radix-4 sequence of n differs from the default one
+constexpr bool has_r4_variant(int n) { return make_plan(n, 4).np != make_plan(n, 8).np; }
 // product of the radices before pass k (the Stockham "Ns")
-constexpr int ns_before(int n, int k) {
-  Plan p = make_plan(n);
+constexpr int ns_before(int n, int k, int rlim = 8) {
+  Plan p = make_plan(n, rlim);
   int s = 1;
   for (int i = 0; i < k; i++) s *= p.r[i];
   return s;
@@ -209,12 +220,12 @@ __device__ __forceinline__ void stockham_pass(const In& in, const Out& out, int 
   if (Out::in_lds) __syncthreads();
 }
 
-template <typename T, int N, int G, int K, typename In, typename Mid, typename Out>
+template <typename T, int N, int G, int K, int RL, typename In, typename Mid, typename Out>
 __device__ __forceinline__ void fft_from(const In& in, const Mid& lds, const Out& out, int p0, int pstride,
                                          int npen, int q, T s, const cx<T>* __restrict__ tw) {
-  constexpr Plan pl = make_plan(N);
+  constexpr Plan pl = make_plan(N, RL);
   if constexpr (K < pl.np) {
-    constexpr int R = pl.r[K], NSB = ns_before(N, K), TPP = pl.tpp;
+    constexpr int R = pl.r[K], NSB = ns_before(N, K, RL), TPP = pl.tpp;
     if constexpr (pl.np == 1) {
       stockham_pass<T, N, R, NSB, TPP, G>(in, out, p0, pstride, npen, q, s, tw);
     } else if constexpr (K == 0) {
@@ -224,7 +235,7 @@ __device__ __forceinline__ void fft_from(const In& in, const Mid& lds, const Out
     } else {
       stockham_pass<T, N, R, NSB, TPP, G>(lds, lds, p0, pstride, npen, q, s, tw);
     }
-    fft_from<T, N, G, K + 1>(in, lds, out, p0, pstride, npen, q, s, tw);
+    fft_from<T, N, G, K + 1, RL>(in, lds, out, p0, pstride, npen, q, s, tw);
   }
 }
 
@@ -233,10 +244,10 @@ __device__ __forceinline__ void fft_from(const In& in, const Mid& lds, const Out
 // `lds` (which needs npen pencils of pitch >= N).  When the input is already
 // in LDS the caller synchronises first; the function ends synchronised when
 // the output is in LDS.
-template <typename T, int N, int G, typename In, typename Mid, typename Out>
+template <typename T, int N, int G, int RL = 8, typename In, typename Mid, typename Out>
 __device__ __forceinline__ void fft_pencils(const In& in, const Mid& lds, const Out& out, int p0,
                                             int pstride, int npen, int q, T s, const cx<T>* __restrict__ tw) {
-  fft_from<T, N, G, 0>(in, lds, out, p0, pstride, npen, q, s, tw);
+  fft_from<T, N, G, 0, RL>(in, lds, out, p0, pstride, npen, q, s, tw);
 }
 
 // All-LDS transforms of up to G pencils per thread (pencil p at sm + p*LD).

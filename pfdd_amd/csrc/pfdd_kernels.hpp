@@ -25,6 +25,8 @@ struct ElasticConsts {
   double mu, ll, mup, inv_mu;      // C44, C12, C11-C12-2C44 (Bmatrix G.c:345-347)
   double sig[MAXS][9];             // s_a = C:e_a
   double SE[MAXS][MAXS];           // s_a:e_b
+  double SEm[MAXS][MAXS];          // s_a:e_b / mu
+  float SEmf[MAXS][MAXS];          // s_a:e_b / mu rounded to FP32
   double Tsum[MAXS];               // sum_ij s_a,ij
   double T[9];                     // T_pq = sum_ij C_pqij
   double C[81];                    // C_ijkl
@@ -119,52 +121,67 @@ constexpr int pow2_floor(int v) {
 }
 // Rows per block of the contiguous-axis kernels: a power of two, so that the
 // block size TILE*TPP is a multiple of 64 whenever it reaches 64.
-template <typename T, int N> constexpr int tile_rows() {
-  constexpr int tpp = make_plan(N).tpp;
+template <typename T, int N, int RL = 8> constexpr int tile_rows() {
+  constexpr int tpp = make_plan(N, RL).tpp;
   int t = pow2_floor(256 / tpp > 0 ? 256 / tpp : 1);
   const int lim = 32768 / (pitch<N>() * (int)sizeof(cx<T>));
   while (t > lim && t > 1) t /= 2;
   return t;
 }
-template <typename T, int N> constexpr int tile_cols() {
-  constexpr int tpp = make_plan(N).tpp;
+// Columns per block of the strided-axis kernel: at least 256 bytes of each
+// row are read together (16 FP64 or 32 FP32 columns), within the LDS and
+// 1024-thread limits.
+template <typename T, int N, int RL = 8> constexpr int tile_cols() {
+  constexpr int tpp = make_plan(N, RL).tpp;
   int t = pow2_floor(256 / tpp > 0 ? 256 / tpp : 1);
-  if (t < 16) t = 16;
+  if (t < 256 / (int)sizeof(cx<T>)) t = 256 / (int)sizeof(cx<T>);
   const int lim = 65536 / (pitch<N>() * (int)sizeof(cx<T>));
   while (t > lim && t > 1) t /= 2;
   while (t * tpp > 1024 && t > 1) t /= 2;
   return t;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Kernels whose neighbouring tiles share cache lines (the strided-axis
+// passes, when a tile row is narrower than a line) give each XCD a
+// contiguous range of tiles instead: block b -> tile xcd_tile(b, nb).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ long xcd_tile(long b, long nb) {
+  const long q = nb / NUM_XCD, r = nb % NUM_XCD;
+  const long x = b % NUM_XCD, i = b / NUM_XCD;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // Transform along the contiguous axis: rows of N consecutive elements.  The
 // first pass reads the rows from HBM, the last writes them back.
-template <typename T, int N>
-__global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
+template <typename T, int N, int RL = 8>
+__global__ void __launch_bounds__((tile_rows<T, N, RL>() * make_plan(N, RL).tpp))
 k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict__ tw) {
-  constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), LD = pitch<N>();
+  constexpr int TPP = make_plan(N, RL).tpp, TILE = tile_rows<T, N, RL>(), LD = pitch<N>();
   __shared__ cx<T> sm[TILE * LD];
   const long row0 = (long)blockIdx.x * TILE;
   const int nr = (int)((nrows - row0) < TILE ? (nrows - row0) : TILE);
   const RowPencils<T> g{data + row0 * N, N};
   const LdsPencils<T> l{sm, LD};
-  fft_pencils<T, N, 1>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, tw);
+  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, tw);
 }
 
 // Transform along a strided axis of an [outer][N][inner] array; each block
 // takes `tile` consecutive inner indices (tile divides inner, tile <= TILE).
 // Threads run pencil-fastest, so a wave's loads of one row of the tile are
 // contiguous.
-template <typename T, int N>
-__global__ void __launch_bounds__((tile_cols<T, N>() * make_plan(N).tpp))
+template <typename T, int N, int RL = 8>
+__global__ void __launch_bounds__((tile_cols<T, N, RL>() * make_plan(N, RL).tpp))
 k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* __restrict__ tw) {
-  constexpr int TILE = tile_cols<T, N>(), LD = pitch<N>();
+  constexpr int TILE = tile_cols<T, N, RL>(), LD = pitch<N>();
   __shared__ cx<T> sm[TILE * LD];
   const long tiles_per_outer = inner / tile;
-  const long o = blockIdx.x / tiles_per_outer;
-  const long i0 = (blockIdx.x % tiles_per_outer) * tile;
+  const long bt = xcd_tile(blockIdx.x, gridDim.x);
+  const long o = bt / tiles_per_outer;
+  const long i0 = (bt % tiles_per_outer) * tile;
   const ColPencils<T> g{data + o * N * inner + i0, inner};
   const LdsPencils<T> l{sm, LD};
-  fft_pencils<T, N, 1>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, tw);
+  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, tw);
 }
 
 // ---------------------------------------------------------------- GL kernels
@@ -208,10 +225,10 @@ __device__ __forceinline__ void gl_point(double2& x, double dre, double dim, con
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 
-// Second pass of the per-field sums: wave w of the single block adds
-// partials[f][0..nblk) for f = w, w + 16, ... in a fixed order (lane-strided,
-// then a butterfly over the wave), so no block-wide barrier sits inside the
-// loop; then each instance's fields (ns consecutive ones) are added in order.
+// Second pass of the per-field sums, one block: each field's partials are
+// added in a fixed order (in up to 16 chunks, each lane-strided and then a
+// butterfly over a wave, then the chunks in order); then each instance's
+// fields (ns consecutive ones) are added in order.
 // Output row: [instance][ns + 1] = the ns field sums and their total.  With
 // step_ctr the row is out + step_ctr[0] * row length and both counters
 // advance (step_ctr[0]: history row, step_ctr[1]: GL steps under the current
@@ -223,20 +240,39 @@ k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns
   const int lane = threadIdx.x & 63;
   const int ninst = nfields / ns;
   const long row = step_ctr ? (long)step_ctr[0] : 0;
-  for (int f = threadIdx.x >> 6; f < nfields; f += FIN_THREADS / 64) {
-    double v = 0.0;
-    for (int b = lane; b < nblk; b += 64) v += partials[(long)f * nblk + b];
+  // a field's partials are cut into nch chunks (by nblk alone, so a field
+  // is added the same way whatever the number of fields); the 16 waves take
+  // (field, chunk) pairs, 16 / nch fields at a time
+  __shared__ double red[FIN_THREADS / 64];
+  constexpr int NW = FIN_THREADS / 64;
+  const int w = threadIdx.x >> 6;
+  const int nch = nblk / 2048 < 1 ? 1 : nblk / 2048 > NW ? NW : nblk / 2048;
+  const int g = NW / nch;
+  const int len = (nblk + nch - 1) / nch;
+  for (int f0 = 0; f0 < nfields; f0 += g) {
+    const int f = f0 + w / nch, ch = w % nch;
+    if (w < g * nch && f < nfields) {
+      const int b0 = ch * len, b1 = b0 + len < nblk ? b0 + len : nblk;
+      double v = 0.0;
+      for (int b = b0 + lane; b < b1; b += 64) v += partials[(long)f * nblk + b];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) fsum[f] = v;
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[w] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < g && f0 + (int)threadIdx.x < nfields) {
+      double t = 0.0;
+      for (int c = 0; c < nch; c++) t += red[threadIdx.x * nch + c];
+      fsum[f0 + threadIdx.x] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   double* dst = out + row * (long)ninst * (ns + 1);
   for (int b = threadIdx.x; b < ninst; b += FIN_THREADS) {
     double tot = 0.0;
-    for (int g = 0; g < ns; g++) {
-      const double v = fsum[b * ns + g];
-      dst[b * (ns + 1) + g] = v;
+    for (int a = 0; a < ns; a++) {
+      const double v = fsum[b * ns + a];
+      dst[b * (ns + 1) + a] = v;
       tot += v;
     }
     dst[b * (ns + 1) + ns] = tot;
@@ -311,14 +347,16 @@ __global__ void k_set_int(int* p, int v) {
 // Contraction at one k-point (G.c:4609-4629): out_a = sum_b B_ab(k) v_b,
 // written to out[a * ostride] (v stays in registers, so out may alias the
 // locations v was read from).  Stored mode reads B from Bsym.  Recompute
-// mode never forms B: with u_a = s_a f (s_a = C:e_a) and w = sum_b u_b v_b,
-//   sum_b B_ab v_b = sum_b (s_a:e_b / mu) v_b - (G u_a) . w / mu,
+// mode never forms B: with u_a = s_a f (s_a = C:e_a), w = sum_b u_b v_b and
+// G symmetric,
+//   sum_b B_ab v_b = sum_b (s_a:e_b / mu) v_b - u_a . (G w) / mu,
 // the same B_ab = (s_a:e_b - u_a.G.u_b)/mu of Bmatrix (G.c:386-402) with the
-// sums regrouped; u and G are evaluated in FP64 and rounded to T.
-// quirk != 0 adds v_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441) in a
-// second pass over a.  UNROLL_A unrolls the loop over outputs (more
-// registers); rolled, v is still only indexed by unrolled loops, so it stays
-// in registers.
+// sums regrouped; u and G/mu are evaluated in FP64 and rounded to T, and the
+// sums run in T.  With FP32 and unrolled outputs the rounded u_a stay in
+// registers; otherwise they are recomputed for the second sum.  quirk != 0 adds v_a * sum_ij DD_a,ij
+// (virtualevolv G.c:1426-1441) in a second pass over a.  UNROLL_A unrolls
+// the loop over outputs (more registers); rolled, v is still only indexed by
+// unrolled loops, so it stays in registers.
 __device__ __forceinline__ void sig_f(const ElasticConsts* __restrict__ ec, int a, const double f[3], double u[3]) {
   const double* s = ec->sig[a];
   u[0] = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
@@ -326,10 +364,18 @@ __device__ __forceinline__ void sig_f(const ElasticConsts* __restrict__ ec, int 
   u[2] = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
 }
 
+template <typename T> __device__ __forceinline__ T se_mu(const ElasticConsts* __restrict__ ec, int a, int b);
+template <> __device__ __forceinline__ double se_mu<double>(const ElasticConsts* __restrict__ ec, int a, int b) {
+  return ec->SEm[a][b];
+}
+template <> __device__ __forceinline__ float se_mu<float>(const ElasticConsts* __restrict__ ec, int a, int b) {
+  return ec->SEmf[a][b];
+}
+
 template <typename T, int SM>
 __device__ __forceinline__ cx<T> contract_row(int a, const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec,
                                               long pt, int S, bool stored, const T* __restrict__ Bsym, long P,
-                                              const double f[3], const double g[6], cx<T> w0, cx<T> w1, cx<T> w2) {
+                                              const double f[3], const T (&ua)[3], const cx<T> (&gw)[3]) {
   cx<T> acc = {(T)0, (T)0};
   if (stored) {
 #pragma unroll
@@ -340,19 +386,15 @@ __device__ __forceinline__ cx<T> contract_row(int a, const cx<T> (&v)[SM], const
       acc.y += v[b].y * bab;
     }
   } else {
-    double u[3], gu[3];
-    sig_f(ec, a, f, u);
-    gmul(g, u, gu);
 #pragma unroll
     for (int b = 0; b < SM; b++) {
       if (b >= S) continue;
-      const T se = (T)(ec->SE[a][b] * ec->inv_mu);
+      const T se = se_mu<T>(ec, a, b);
       acc.x += v[b].x * se;
       acc.y += v[b].y * se;
     }
-    const T g0 = (T)(gu[0] * ec->inv_mu), g1 = (T)(gu[1] * ec->inv_mu), g2 = (T)(gu[2] * ec->inv_mu);
-    acc.x -= g0 * w0.x + g1 * w1.x + g2 * w2.x;
-    acc.y -= g0 * w0.y + g1 * w1.y + g2 * w2.y;
+    acc.x -= ua[0] * gw[0].x + ua[1] * gw[1].x + ua[2] * gw[2].x;
+    acc.y -= ua[0] * gw[0].y + ua[1] * gw[1].y + ua[2] * gw[2].y;
   }
   return acc;
 }
@@ -361,6 +403,8 @@ template <typename T, int SM, bool UNROLL_A>
 __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec, long pt,
                                                int S, int quirk, bool stored, const T* __restrict__ Bsym, long P,
                                                cx<T>* out, long ostride) {
+  // rounded u_a kept in registers (FP32, outputs unrolled) or recomputed
+  constexpr bool KEEP_U = sizeof(T) == 4 && UNROLL_A;
   double f[3], g[6];
   kvec(pt, ec, f);
   if (!green6(f, ec, g)) {  // B(0) = 0 and DD(0) = 0
@@ -369,27 +413,59 @@ __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const Elast
       if (a < S) out[a * ostride] = {(T)0, (T)0};
     return;
   }
-  cx<T> w0 = {(T)0, (T)0}, w1 = {(T)0, (T)0}, w2 = {(T)0, (T)0};
+  T uk[KEEP_U ? SM : 1][3];
+  cx<T> gw[3] = {{(T)0, (T)0}, {(T)0, (T)0}, {(T)0, (T)0}};
   if (!stored) {
+    cx<T> w[3] = {{(T)0, (T)0}, {(T)0, (T)0}, {(T)0, (T)0}};
 #pragma unroll
     for (int b = 0; b < SM; b++) {
       if (b >= S) continue;
       double ud[3];
       sig_f(ec, b, f, ud);
       const T u0 = (T)ud[0], u1 = (T)ud[1], u2 = (T)ud[2];
-      w0.x += u0 * v[b].x; w0.y += u0 * v[b].y;
-      w1.x += u1 * v[b].x; w1.y += u1 * v[b].y;
-      w2.x += u2 * v[b].x; w2.y += u2 * v[b].y;
+      if constexpr (KEEP_U) {
+        uk[b][0] = u0;
+        uk[b][1] = u1;
+        uk[b][2] = u2;
+      }
+      w[0].x += u0 * v[b].x; w[0].y += u0 * v[b].y;
+      w[1].x += u1 * v[b].x; w[1].y += u1 * v[b].y;
+      w[2].x += u2 * v[b].x; w[2].y += u2 * v[b].y;
     }
+    // gw = (G / mu) w
+    const T g0 = (T)(g[0] * ec->inv_mu), g1 = (T)(g[1] * ec->inv_mu), g2 = (T)(g[2] * ec->inv_mu);
+    const T g3 = (T)(g[3] * ec->inv_mu), g4 = (T)(g[4] * ec->inv_mu), g5 = (T)(g[5] * ec->inv_mu);
+    gw[0].x = g0 * w[0].x + g1 * w[1].x + g2 * w[2].x;
+    gw[0].y = g0 * w[0].y + g1 * w[1].y + g2 * w[2].y;
+    gw[1].x = g1 * w[0].x + g3 * w[1].x + g4 * w[2].x;
+    gw[1].y = g1 * w[0].y + g3 * w[1].y + g4 * w[2].y;
+    gw[2].x = g2 * w[0].x + g4 * w[1].x + g5 * w[2].x;
+    gw[2].y = g2 * w[0].y + g4 * w[1].y + g5 * w[2].y;
   }
+  auto row = [&](int a) -> cx<T> {
+    T ua[3] = {(T)0, (T)0, (T)0};
+    if (!stored) {
+      if constexpr (KEEP_U) {
+        ua[0] = uk[a][0];
+        ua[1] = uk[a][1];
+        ua[2] = uk[a][2];
+      } else {
+        double ud[3];
+        sig_f(ec, a, f, ud);
+        ua[0] = (T)ud[0];
+        ua[1] = (T)ud[1];
+        ua[2] = (T)ud[2];
+      }
+    }
+    return contract_row<T, SM>(a, v, ec, pt, S, stored, Bsym, P, f, ua, gw);
+  };
   if (UNROLL_A) {
 #pragma unroll
     for (int a = 0; a < SM; a++)
-      if (a < S) out[a * ostride] = contract_row<T, SM>(a, v, ec, pt, S, stored, Bsym, P, f, g, w0, w1, w2);
+      if (a < S) out[a * ostride] = row(a);
   } else {
 #pragma unroll 1
-    for (int a = 0; a < S; a++)
-      out[a * ostride] = contract_row<T, SM>(a, v, ec, pt, S, stored, Bsym, P, f, g, w0, w1, w2);
+    for (int a = 0; a < S; a++) out[a * ostride] = row(a);
   }
   if (quirk) {
     double tf[3];
@@ -483,16 +559,19 @@ k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __res
 // forward N3 and N2 passes (the last one writes HBM): with the N1 passes and
 // the contraction this is the whole GL step in two HBM round trips of W.
 // Block sum of Re xi / P -> partials[f][i1].
-constexpr int PLANE_THREADS = 1024;
 template <typename T, int N2, int N3>
 constexpr size_t plane_lds_bytes() { return (size_t)N2 * (N3 + 1) * sizeof(cx<T>); }
+// 1024 threads, or 512 for the 128^2 FP32 plane, whose passes would not fit
+// in the 128 registers per thread that a 1024-thread block allows.
+template <typename T, int N2, int N3>
+constexpr int plane_threads() { return (sizeof(T) == 4 && N2 * N3 >= 16384) ? 512 : 1024; }
 
 template <typename T, int N2, int N3, bool MASK, bool TAUPP>
-__global__ void __launch_bounds__(PLANE_THREADS)
+__global__ void __launch_bounds__((plane_threads<T, N2, N3>()))
 k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
              const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
              const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
-  constexpr int LD = N3 + 1, PLANE = N2 * N3;
+  constexpr int LD = N3 + 1, PLANE = N2 * N3, PLANE_THREADS = plane_threads<T, N2, N3>();
   constexpr int TPP2 = make_plan(N2).tpp, TPP3 = make_plan(N3).tpp;
   constexpr int PPI2 = PLANE_THREADS / TPP2, PPI3 = PLANE_THREADS / TPP3;
   constexpr int G2 = (N3 + PPI2 - 1) / PPI2, G3 = (N2 + PPI3 - 1) / PPI3;  // N3 column pencils, N2 row pencils
@@ -510,24 +589,51 @@ k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __r
   // threads past PPI*TPP (when TPP does not divide the block) sit out
   const int c_p0 = t < PPI2 * TPP2 ? t % PPI2 : (1 << 30), c_q = t / PPI2;
   const int r_p0 = t < PPI3 * TPP3 ? t / TPP3 : (1 << 30), r_q = t % TPP3;
+  double2* xb = xi + off;
+  // All of this thread's xi values are loaded in one batch, so
+  // their latency is paid once (one block per CU leaves no other block to
+  // hide it).  Up to 4 points per thread the batch is issued before the
+  // inverse passes and lands while they run; larger batches would not fit
+  // in registers beside the passes and are issued after them.
+  // (The masked 96^2 FP64 variant has no registers to spare for the batch.)
+  constexpr int EPT = (PLANE + PLANE_THREADS - 1) / PLANE_THREADS;
+  constexpr bool EARLY = EPT <= 4;
+  constexpr bool BATCH = EARLY || !MASK || sizeof(T) == 4;
+  double2 xv[EPT];
+  auto fetch = [&]() {
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+      const int e = t + k * PLANE_THREADS;
+      if (BATCH && e < PLANE) xv[k] = xb[e];
+    }
+  };
+  if (EARLY) fetch();
   // inverse along N2 (columns: pencil i3, pencil-fastest threads), HBM -> LDS
   fft_pencils<T, N2, G2>(gcol, lcol, lcol, c_p0, PPI2, N3, c_q, (T)1, tw2);
   // inverse along N3 (rows)
   fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)1, tw3);
+  if (!EARLY) fetch();
   const double nsize = (double)P;
   const FieldCoef c = field_coef(a, f, step_ctr);
-  double2* xb = xi + off;
   double acc = 0.0;
-  for (int e = t; e < PLANE; e += PLANE_THREADS) {
+  auto point = [&](int e, double2 x) {
     const int li = (e / N3) * LD + (e % N3);
     const cx<T> d = sm[li];
-    double2 x = xb[e];
     if (!MASK || mask[off + e] == 0) {
       gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, (long)i1 * PLANE + e, nsize);
       xb[e] = x;
     }
     sm[li] = {(T)x.x, (T)x.y};
     acc += x.x / nsize;
+  };
+  if constexpr (BATCH) {
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+      const int e = t + k * PLANE_THREADS;
+      if (e < PLANE) point(e, xv[k]);
+    }
+  } else {
+    for (int e = t; e < PLANE; e += PLANE_THREADS) point(e, xb[e]);
   }
   __syncthreads();
   fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)-1, tw3);
@@ -559,7 +665,7 @@ k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   const int S = ec->ns;
   W += (long)blockIdx.y * S * P;  // instance
   const long inner = P / N;
-  const long m0 = (long)blockIdx.x * tile;
+  const long m0 = xcd_tile(blockIdx.x, gridDim.x) * tile;
   const int npen = S * tile;
   const int per_field = N * tile;
   for (int e = threadIdx.x; e < S * per_field; e += C3_THREADS) {

@@ -208,3 +208,22 @@ def test_ensemble_autotune_then_oracle():
     for m in members:
         xo, _ = oracle_run(w, m, 40)
         assert np.abs(got[m.index] - xo).max() <= 1e-10, m.index
+
+
+def test_time_step_kernels_follows_the_plan_and_advances_state():
+    """The in-sequence kernel timing runs real GL steps: its launch counts are
+    the step plan's, and the state it leaves equals that of pfdd_gl_steps."""
+    w = small_ensemble(count=1, sizes=(32,))
+    m = w.member(0)
+    for plan in (pa.PLAN_AUTO, pa.PLAN_NO_PLANE, pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_ROWS | pa.PLAN_UNFUSED_COLS):
+        with pa.Context(w.params(0, pa.FP64, plan=plan)) as c:
+            c.set_tau(m.tau)
+            c.upload_xi(m.xi)
+            c.begin_outer(True)
+            t = c.time_step_kernels(5)
+            steps_plan = c.step_plan()
+            for st, (ms, n) in t.items():
+                assert ms > 0 and n == steps_plan[st], (plan, st)
+            x = c.download_xi()
+        xs, _ = single_run(w, m, 5, pa.FP64, plan=plan)
+        assert np.array_equal(x, xs), plan

@@ -76,6 +76,31 @@ def test_fft_matches_oracle(shape, prec):
             assert err < tol, (shape, sign, err)
 
 
+@pytest.mark.parametrize("shape", [(128, 128, 128), (48, 48, 48), (96, 32, 16), (64, 192, 12), (512, 8, 2),
+                                   (16, 24, 8)])
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+def test_fft_radix4_sequences_match_oracle(shape, prec):
+    """The radix-4 pass sequences (a plan choice of the autotuner) compute the
+    same transforms."""
+    say(f"fft radix-4 {shape} prec={prec}")
+    rng = np.random.default_rng(sum(shape) + 7)
+    x = rng.standard_normal((2,) + shape) + 1j * rng.standard_normal((2,) + shape)
+    eps = ph.fcc_eps([0])
+    w = C.Workload("fft", shape, eps, np.zeros((1, 3)), np.zeros((1,) + shape, complex), np.zeros(1),
+                   np.zeros((3, 3)), 0)
+    dt = np.complex128 if prec == pa.FP64 else np.complex64
+    tol = 1e-13 if prec == pa.FP64 else 3e-6
+    plan = pa.PLAN_RADIX4_AXIS1 | pa.PLAN_RADIX4_AXIS2 | pa.PLAN_RADIX4_AXIS3
+    with pa.Context(w.params(prec, pa.B_RECOMPUTE, plan)) as c:
+        for sign in (pa.FORWARD, pa.BACKWARD):
+            buf = hip.DeviceBuffer.from_array(x.astype(dt))
+            c.fft(buf.ptr, 2, sign, prec)
+            c.sync()
+            got = buf.to_array(dt, x.shape)
+            want = O.fft3d(x.astype(dt).astype(np.complex128), sign)
+            assert np.abs(got - want).max() / np.abs(want).max() < tol, (shape, sign)
+
+
 def test_fft_round_trip_256_cubed():
     say("fft round trip 256^3 fp32")
     shape = (256, 256, 256)
@@ -387,7 +412,10 @@ def test_fft_host_seam_matches_reference_fourn():
 @pytest.mark.parametrize("plan", [pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_ROWS, pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_COLS,
                                   pa.PLAN_NO_PLANE | pa.PLAN_FUSED_COLS,
                                   pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_ROWS | pa.PLAN_FUSED_COLS,
-                                  pa.PLAN_FUSED_COLS, pa.PLAN_UNFUSED_COLS])
+                                  pa.PLAN_FUSED_COLS, pa.PLAN_UNFUSED_COLS,
+                                  pa.PLAN_NO_PLANE | pa.PLAN_UNFUSED_ROWS | pa.PLAN_UNFUSED_COLS
+                                  | pa.PLAN_RADIX4_AXIS1 | pa.PLAN_RADIX4_AXIS2 | pa.PLAN_RADIX4_AXIS3,
+                                  pa.PLAN_UNFUSED_COLS | pa.PLAN_RADIX4_AXIS1])
 @pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
 def test_kernel_plans_agree(plan, prec):
     """Every kernel plan computes the same GL step (fused or unfused passes)."""
