@@ -222,10 +222,27 @@ int pfdd_ensemble_begin_outer(pfdd_ens *e, int quirk);
 /* nsteps GL steps of every instance; gamma[nsteps][count] (may be NULL)
  * receives each instance's gamma.  Synchronous when gamma != NULL. */
 int pfdd_ensemble_gl_steps(pfdd_ens *e, int nsteps, double *gamma);
-/* pfdd_autotune for each class; plans[nclasses] and ms_per_step[nclasses]
- * may be NULL.  pfdd_ensemble_begin_outer must run again afterwards. */
+/* pfdd_autotune for each class, then the choice of mode (below);
+ * plans[nclasses] and ms_per_step[nclasses] may be NULL.  xi is restored;
+ * pfdd_ensemble_begin_outer must run again afterwards. */
 int pfdd_ensemble_autotune(pfdd_ens *e, int steps, int *plans, float *ms_per_step);
 int pfdd_ensemble_set_plan(pfdd_ens *e, int cls, int plan);
+/* How a GL step of the ensemble runs (no reference counterpart):
+ * PFDD_ENS_PHASED: every class's kernels, GRAPH_STEPS steps of all classes
+ * captured in one hipGraph; PFDD_ENS_PERSISTENT: one persistent cooperative
+ * launch per step and precision that takes the plane items of every class
+ * from a work counter, then (after a grid barrier) the gamma finish and the
+ * fused column items.  Persistent mode needs every class cubic with
+ * N in {32,48,64,96}, ns <= 4, recomputed B, no frozen points and uniform
+ * shear (PFDD_E_UNSUPPORTED otherwise); it sets each class's plan to
+ * PFDD_PLAN_PLANE | PFDD_PLAN_FUSED_COLS.  Both modes give the same bits.
+ * pfdd_ensemble_begin_outer must run again after a change.
+ * pfdd_ensemble_autotune tunes the class plans, then times both modes and
+ * keeps the faster; pfdd_ensemble_get_mode returns the mode and the two
+ * times per step of that comparison (0 when not measured). */
+enum { PFDD_ENS_PHASED = 0, PFDD_ENS_PERSISTENT = 1 };
+int pfdd_ensemble_set_mode(pfdd_ens *e, int mode);
+int pfdd_ensemble_get_mode(const pfdd_ens *e, float *ms_phased, float *ms_persistent);
 int pfdd_ensemble_sync(pfdd_ens *e);
 void *pfdd_ensemble_stream(pfdd_ens *e);
 long long pfdd_ensemble_device_bytes(const pfdd_ens *e);

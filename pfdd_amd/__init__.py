@@ -55,8 +55,10 @@ EXPORTED = [
     "pfdd_autotune", "pfdd_create_ensemble", "pfdd_destroy_ensemble", "pfdd_ensemble_info",
     "pfdd_ensemble_set_tau", "pfdd_ensemble_upload_xi", "pfdd_ensemble_download_xi", "pfdd_ensemble_begin_outer",
     "pfdd_ensemble_gl_steps", "pfdd_ensemble_autotune", "pfdd_ensemble_set_plan", "pfdd_ensemble_sync",
-    "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes", "pfdd_time_step_kernels",
+    "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes", "pfdd_time_step_kernels", "pfdd_ensemble_set_mode",
+    "pfdd_ensemble_get_mode",
 ]
+ENS_PHASED, ENS_PERSISTENT = 0, 1
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
  STAGE_FINISH) = range(8)
 
@@ -159,6 +161,8 @@ def load():
     L.pfdd_ensemble_stream.restype = vp
     L.pfdd_ensemble_device_bytes.argtypes = [vp]
     L.pfdd_ensemble_device_bytes.restype = ctypes.c_longlong
+    L.pfdd_ensemble_set_mode.argtypes = [vp, ci]
+    L.pfdd_ensemble_get_mode.argtypes = [vp, fp, fp]
     _lib = L
     return L
 
@@ -395,6 +399,17 @@ class Ensemble:
 
     def set_plan(self, cls, plan):
         _check(load().pfdd_ensemble_set_plan(self.h, cls, plan), "pfdd_ensemble_set_plan")
+
+    def set_mode(self, mode):
+        _check(load().pfdd_ensemble_set_mode(self.h, mode), "pfdd_ensemble_set_mode")
+
+    def mode(self):
+        """(mode, ms per step phased, ms per step persistent) of the last autotune comparison."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        m = load().pfdd_ensemble_get_mode(self.h, ctypes.byref(a), ctypes.byref(b))
+        if m < 0:
+            raise PfddError(m, "pfdd_ensemble_get_mode")
+        return m, a.value, b.value
 
     def sync(self):
         _check(load().pfdd_ensemble_sync(self.h), "pfdd_ensemble_sync")

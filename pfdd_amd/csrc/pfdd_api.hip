@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <functional>
 #include <new>
 #include <vector>
 
@@ -977,7 +978,7 @@ int begin_outer(pfdd_ctx* c, int quirk) {
 // steps, history rows) receives the gamma history of each context.
 template <typename Sink>
 int run_steps(pfdd_ctx* const* cs, int nc, hipStream_t st, hipGraphExec_t* graph, int nsteps, bool want_hist,
-              Sink&& sink) {
+              Sink&& sink, const std::function<int()>* step_all = nullptr) {
   int done = 0;
   std::vector<double> h;
   while (done < nsteps) {
@@ -986,7 +987,8 @@ int run_steps(pfdd_ctx* const* cs, int nc, hipStream_t st, hipGraphExec_t* graph
       hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, cs[i]->step_ctr, 0);
     HIPCK(hipPeekAtLastError());
     int left = chunk;
-    if (left >= GRAPH_STEPS && !*graph) {
+    // (a step_all, one cooperative launch per step, is not captured)
+    if (!step_all && left >= GRAPH_STEPS && !*graph) {
       hipGraph_t g;
       HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       int rc = PFDD_OK;
@@ -999,14 +1001,19 @@ int run_steps(pfdd_ctx* const* cs, int nc, hipStream_t st, hipGraphExec_t* graph
       (void)hipGraphDestroy(g);
       HIPCK(ec);
     }
-    while (left >= GRAPH_STEPS && *graph) {
+    while (!step_all && left >= GRAPH_STEPS && *graph) {
       HIPCK(hipGraphLaunch(*graph, st));
       left -= GRAPH_STEPS;
     }
     while (left > 0) {
-      for (int i = 0; i < nc; i++) {
-        int rc = step(cs[i]);
+      if (step_all) {
+        const int rc = (*step_all)();
         if (rc) return rc;
+      } else {
+        for (int i = 0; i < nc; i++) {
+          const int rc = step(cs[i]);
+          if (rc) return rc;
+        }
       }
       left--;
     }
@@ -1382,6 +1389,17 @@ extern "C" int pfdd_enqueue_stage(pfdd_ctx* c, int stage, int reps) {
 // applied shear, which is set per instance) form a class; each class is one
 // batched context, and every class runs on the ensemble's stream, so that
 // GRAPH_STEPS steps of the whole ensemble are one captured graph.
+namespace {
+// The persistent step of the classes of one precision (k_ens_step).
+struct PersGroup {
+  EnsDev* dev = nullptr;
+  int* q = nullptr;
+  int nc = 0;
+  int grid = 0;
+  size_t lds = 0;
+};
+}  // namespace
+
 struct pfdd_ens {
   int count = 0;
   hipStream_t stream = nullptr;
@@ -1389,6 +1407,10 @@ struct pfdd_ens {
   std::vector<int> cls_of, slot_of;  // per instance
   hipGraphExec_t graph = nullptr;
   int prologue_done = 0;
+  int mode = PFDD_ENS_PHASED;
+  PersGroup pg[2];                   // [0] FP32 classes, [1] FP64 classes
+  int pers_built = 0;
+  float ms_phased = 0.f, ms_persistent = 0.f;  // last autotune measurement of each mode
 };
 
 namespace {
@@ -1418,6 +1440,119 @@ void ens_drop_graph(pfdd_ens* e) {
 
 bool ens_inst_ok(const pfdd_ens* e, int inst) { return e && inst >= 0 && inst < e->count; }
 
+bool pers_len(int n) { return n == 32 || n == 48 || n == 64 || n == 96; }
+
+// A class the persistent step can run (see k_ens_step).
+bool pers_eligible(const pfdd_ctx* c) {
+  return c->S <= 4 && !c->stored && !c->mask && !c->taupp && c->p.n1 == c->p.n2 && c->p.n2 == c->p.n3 &&
+         pers_len(c->p.n1);
+}
+
+size_t plane_lds_of(int n, bool fp64) {
+  switch (n) {
+#define PFDD_L(NN) case NN: return fp64 ? plane_lds_bytes<double, NN, NN>() : plane_lds_bytes<float, NN, NN>();
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96)
+#undef PFDD_L
+    default: return 0;
+  }
+}
+
+template <typename T>
+const void* ens_kernel() {
+  return reinterpret_cast<const void*>(&k_ens_step<T>);
+}
+
+// Work lists and launch shape of the persistent step, per precision.
+int pers_build(pfdd_ens* e) {
+  int dev = 0, ncu = 0, coop = 0;
+  HIPCK(hipGetDevice(&dev));
+  HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) return PFDD_E_UNSUPPORTED;
+  for (int fp = 0; fp < 2; fp++) {
+    std::vector<pfdd_ctx*> cl;
+    for (pfdd_ctx* c : e->cls)
+      if (c->fp64 == fp) cl.push_back(c);
+    PersGroup& g = e->pg[fp];
+    g.nc = (int)cl.size();
+    if (cl.empty()) continue;
+    if ((int)cl.size() > ENS_MAX_CLASSES) return PFDD_E_UNSUPPORTED;
+    // larger grids first, so the long items start early
+    std::stable_sort(cl.begin(), cl.end(), [](const pfdd_ctx* a, const pfdd_ctx* b) { return a->p.n1 > b->p.n1; });
+    EnsDev h{};
+    h.nc = (int)cl.size();
+    long plane = 0, col = h.nc;
+    size_t lds = 0;
+    for (int k = 0; k < h.nc; k++) {
+      const pfdd_ctx* c = cl[k];
+      if (!pers_eligible(c) || !c->plane || c->c3_tile <= 0) return PFDD_E_UNSUPPORTED;
+      EnsClassDev& d = h.cls[k];
+      d.W = c->W;
+      d.xi = c->xi;
+      d.ec = c->ec;
+      for (int ax = 0; ax < 3; ax++) d.tw[ax] = fp ? (const void*)c->tw[ax].d : (const void*)c->tw[ax].f;
+      d.sa = c->sa;
+      d.partials = c->partials;
+      d.hist = c->hist;
+      d.step_ctr = c->step_ctr;
+      d.P = c->P;
+      d.n = c->p.n1;
+      d.F = c->F;
+      d.B = c->B;
+      d.S = c->S;
+      d.c3_tile = c->c3_tile;
+      d.plane_first = plane;
+      d.col_first = col;
+      plane += (long)c->F * c->p.n1;
+      col += (long)c->B * ((long)c->p.n2 * c->p.n3 / c->c3_tile);
+      lds = std::max(lds, std::max(plane_lds_of(c->p.n1, fp), std::max(c->c3_lds, sizeof(double) * c->F)));
+    }
+    h.n_plane = plane;
+    h.n_col = col;
+    if (!g.q) {
+      if (hipMalloc(&g.q, sizeof(int) * 2) != hipSuccess) return PFDD_E_ALLOC;
+      HIPCK(hipMemsetAsync(g.q, 0, sizeof(int) * 2, e->stream));
+    }
+    h.q = g.q;
+    if (!g.dev && hipMalloc(&g.dev, sizeof(EnsDev)) != hipSuccess) return PFDD_E_ALLOC;
+    HIPCK(hipMemcpyAsync(g.dev, &h, sizeof(EnsDev), hipMemcpyHostToDevice, e->stream));
+    HIPCK(hipStreamSynchronize(e->stream));
+    const void* kern = fp ? ens_kernel<double>() : ens_kernel<float>();
+    HIPCK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, ENS_THREADS, lds));
+    if (per_cu < 1) return PFDD_E_UNSUPPORTED;
+    g.grid = per_cu * ncu;
+    g.lds = lds;
+  }
+  e->pers_built = 1;
+  return PFDD_OK;
+}
+
+// One GL step of every class: one cooperative launch per precision.
+int pers_step(pfdd_ens* e) {
+  for (int fp = 0; fp < 2; fp++) {
+    PersGroup& g = e->pg[fp];
+    if (!g.nc) continue;
+    void* args[] = {&g.dev};
+    const void* kern = fp ? ens_kernel<double>() : ens_kernel<float>();
+    HIPCK(hipLaunchCooperativeKernel(kern, dim3((unsigned)g.grid), dim3(ENS_THREADS), args, (unsigned)g.lds,
+                                     e->stream));
+  }
+  return PFDD_OK;
+}
+
+// Plans the persistent step needs (plane kernel + fused columns).
+int pers_plans(pfdd_ens* e) {
+  for (pfdd_ctx* c : e->cls) {
+    if (!pers_eligible(c)) return PFDD_E_UNSUPPORTED;
+    const int rc = apply_plan(c, PFDD_PLAN_PLANE | PFDD_PLAN_FUSED_COLS);
+    if (rc) return rc;
+    if (!c->plane || c->c3_tile <= 0) return PFDD_E_UNSUPPORTED;
+  }
+  return PFDD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1426,6 +1561,10 @@ void pfdd_destroy_ensemble(pfdd_ens* e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  for (PersGroup& g : e->pg) {
+    if (g.dev) (void)hipFree(g.dev);
+    if (g.q) (void)hipFree(g.q);
+  }
   for (pfdd_ctx* c : e->cls) pfdd_destroy(c);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -1520,6 +1659,16 @@ int pfdd_ensemble_gl_steps(pfdd_ens* e, int nsteps, double* gamma) {
   std::vector<std::vector<int>> members(e->cls.size());
   for (int i = 0; i < e->count; i++) members[e->cls_of[i]].push_back(i);
   const int count = e->count;
+  std::function<int()> pstep;
+  if (e->mode == PFDD_ENS_PERSISTENT) {
+    for (pfdd_ctx* c : e->cls)
+      if (!pers_eligible(c)) return PFDD_E_UNSUPPORTED;  // e.g. a per-point shear set since
+    if (!e->pers_built) {
+      const int rc = pers_build(e);
+      if (rc) return rc;
+    }
+    pstep = [e]() { return pers_step(e); };
+  }
   return run_steps(e->cls.data(), (int)e->cls.size(), e->stream, &e->graph, nsteps, gamma != nullptr,
                    [&](int k, int first, int n, const double* h) {
                      const int S = e->cls[k]->S;
@@ -1527,22 +1676,134 @@ int pfdd_ensemble_gl_steps(pfdd_ens* e, int nsteps, double* gamma) {
                      for (int s = 0; s < n; s++)
                        for (size_t b = 0; b < members[k].size(); b++)
                          gamma[(size_t)(first + s) * count + members[k][b]] = h[(size_t)s * row + b * (S + 1) + S];
-                   });
+                   },
+                   e->mode == PFDD_ENS_PERSISTENT ? &pstep : nullptr);
+}
+
+int pfdd_ensemble_set_mode(pfdd_ens* e, int mode) {
+  if (!e || (mode != PFDD_ENS_PHASED && mode != PFDD_ENS_PERSISTENT)) return PFDD_E_SHAPE;
+  HIPCK(hipStreamSynchronize(e->stream));
+  ens_drop_graph(e);
+  e->prologue_done = 0;
+  e->pers_built = 0;
+  if (mode == PFDD_ENS_PERSISTENT) {
+    const int rc = pers_plans(e);
+    if (rc) {
+      for (pfdd_ctx* c : e->cls) (void)apply_plan(c, c->p.plan);
+      e->mode = PFDD_ENS_PHASED;
+      return rc;
+    }
+  }
+  e->mode = mode;
+  return PFDD_OK;
+}
+
+int pfdd_ensemble_get_mode(const pfdd_ens* e, float* ms_phased, float* ms_persistent) {
+  if (!e) return PFDD_E_SHAPE;
+  if (ms_phased) *ms_phased = e->ms_phased;
+  if (ms_persistent) *ms_persistent = e->ms_persistent;
+  return e->mode;
 }
 
 int pfdd_ensemble_autotune(pfdd_ens* e, int steps, int* plans, float* ms_per_step) {
   if (!e) return PFDD_E_SHAPE;
-  ens_drop_graph(e);
-  e->prologue_done = 0;
+  int rc = pfdd_ensemble_set_mode(e, PFDD_ENS_PHASED);
+  if (rc) return rc;
+  std::vector<int> tuned(e->cls.size());
   for (size_t k = 0; k < e->cls.size(); k++) {
-    int plan = 0;
     float ms = 0.f;
-    const int rc = autotune_ctx(e->cls[k], steps, &plan, &ms);
-    if (rc) return rc;
-    if (plans) plans[k] = plan;
+    if ((rc = autotune_ctx(e->cls[k], steps, &tuned[k], &ms))) return rc;
+    if (plans) plans[k] = tuned[k];
     if (ms_per_step) ms_per_step[k] = ms;
   }
-  return PFDD_OK;
+  // the mode: the tuned phased plans against the persistent step
+  e->ms_phased = e->ms_persistent = 0.f;
+  bool elig = true;
+  for (pfdd_ctx* c : e->cls) elig = elig && pers_eligible(c);
+  if (!elig) return PFDD_OK;
+  int coop = 0, dev = 0;
+  HIPCK(hipGetDevice(&dev));
+  HIPCK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) return PFDD_OK;
+  std::vector<void*> xsave(e->cls.size(), nullptr);
+  std::vector<int*> csave(e->cls.size(), nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const int timed = ((std::max(steps, 1) + GRAPH_STEPS - 1) / GRAPH_STEPS) * GRAPH_STEPS;
+  auto restore = [&]() -> int {
+    for (size_t k = 0; k < e->cls.size(); k++) {
+      pfdd_ctx* c = e->cls[k];
+      HIPCK(hipMemcpyAsync(c->xi, xsave[k], sizeof(double2) * (size_t)c->F * c->P, hipMemcpyDeviceToDevice,
+                           e->stream));
+      HIPCK(hipMemcpyAsync(c->step_ctr, csave[k], sizeof(int) * 4, hipMemcpyDeviceToDevice, e->stream));
+    }
+    HIPCK(hipStreamSynchronize(e->stream));
+    return PFDD_OK;
+  };
+  auto time_mode = [&](int mode, float* best) -> int {
+    int r;
+    if (mode == PFDD_ENS_PHASED) {
+      if ((r = pfdd_ensemble_set_mode(e, PFDD_ENS_PHASED))) return r;
+      for (size_t k = 0; k < e->cls.size(); k++)
+        if ((r = apply_plan(e->cls[k], tuned[k]))) return r;
+    } else if ((r = pfdd_ensemble_set_mode(e, PFDD_ENS_PERSISTENT))) {
+      return r;
+    }
+    if ((r = pfdd_ensemble_begin_outer(e, 0))) return r;
+    if ((r = pfdd_ensemble_gl_steps(e, GRAPH_STEPS, nullptr))) return r;
+    HIPCK(hipEventRecord(e0, e->stream));
+    if ((r = pfdd_ensemble_gl_steps(e, timed, nullptr))) return r;
+    HIPCK(hipEventRecord(e1, e->stream));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    ms /= (float)timed;
+    if (*best == 0.f || ms < *best) *best = ms;
+    return restore();
+  };
+  bool saved = false;
+  do {
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
+    for (size_t k = 0; k < e->cls.size() && rc == PFDD_OK; k++) {
+      pfdd_ctx* c = e->cls[k];
+      const size_t xb = sizeof(double2) * (size_t)c->F * c->P;
+      if (hipMalloc(&xsave[k], xb) != hipSuccess || hipMalloc(&csave[k], sizeof(int) * 4) != hipSuccess) {
+        rc = PFDD_E_ALLOC;
+        break;
+      }
+      if (hipMemcpyAsync(xsave[k], c->xi, xb, hipMemcpyDeviceToDevice, e->stream) != hipSuccess ||
+          hipMemcpyAsync(csave[k], c->step_ctr, sizeof(int) * 4, hipMemcpyDeviceToDevice, e->stream) != hipSuccess)
+        rc = PFDD_E_HIP;
+    }
+    if (rc || hipStreamSynchronize(e->stream) != hipSuccess) {
+      if (!rc) rc = PFDD_E_HIP;
+      break;
+    }
+    saved = true;
+    for (int round = 0; round < 2 && rc == PFDD_OK; round++) {
+      if ((rc = time_mode(PFDD_ENS_PHASED, &e->ms_phased))) break;
+      rc = time_mode(PFDD_ENS_PERSISTENT, &e->ms_persistent);
+    }
+  } while (0);
+  if (saved) {
+    const int r = restore();
+    if (rc == PFDD_OK) rc = r;
+  }
+  // keep the faster mode (the phased one on any failure of the persistent step)
+  if (rc == PFDD_OK && e->ms_persistent > 0.f && e->ms_persistent < e->ms_phased) {
+    rc = pfdd_ensemble_set_mode(e, PFDD_ENS_PERSISTENT);
+  } else {
+    const int r = pfdd_ensemble_set_mode(e, PFDD_ENS_PHASED);
+    for (size_t k = 0; k < e->cls.size(); k++) (void)apply_plan(e->cls[k], tuned[k]);
+    if (rc == PFDD_OK) rc = r;
+  }
+  for (void* x : xsave)
+    if (x) (void)hipFree(x);
+  for (int* x : csave)
+    if (x) (void)hipFree(x);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  e->prologue_done = 0;
+  return rc;
 }
 
 int pfdd_ensemble_set_plan(pfdd_ens* e, int cls, int plan) {

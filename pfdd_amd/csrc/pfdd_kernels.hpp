@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "pfdd_fft.hpp"
 
 namespace pfdd {
@@ -234,9 +236,9 @@ constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 // advance (step_ctr[0]: history row, step_ctr[1]: GL steps under the current
 // tau ramp).  nfields <= FIN_MAX_FIELDS, a multiple of ns.
 constexpr int FIN_THREADS = 1024, FIN_MAX_FIELDS = 4096;
-__global__ void __launch_bounds__(FIN_THREADS)
-k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns, double* out, int* step_ctr) {
-  __shared__ double fsum[FIN_MAX_FIELDS];
+// The block's work (FIN_THREADS threads); fsum: nfields doubles of LDS.
+__device__ __forceinline__ void finish_body(const double* __restrict__ partials, int nfields, int nblk, int ns,
+                                            double* out, int* step_ctr, double* fsum) {
   const int lane = threadIdx.x & 63;
   const int ninst = nfields / ns;
   const long row = step_ctr ? (long)step_ctr[0] : 0;
@@ -281,6 +283,12 @@ k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns
     step_ctr[0] = step_ctr[0] + 1;  // history row (read by every thread before the barrier above)
     step_ctr[1] = step_ctr[1] + 1;  // GL steps since the tau ramp was set
   }
+}
+
+__global__ void __launch_bounds__(FIN_THREADS)
+k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns, double* out, int* step_ctr) {
+  __shared__ double fsum[FIN_MAX_FIELDS];
+  finish_body(partials, nfields, nblk, ns, out, step_ctr, fsum);
 }
 
 // plasticevolv (G.c:3470-3501) fused with the copy xi -> data (G.c:4581-4590)
@@ -566,21 +574,21 @@ constexpr size_t plane_lds_bytes() { return (size_t)N2 * (N3 + 1) * sizeof(cx<T>
 template <typename T, int N2, int N3>
 constexpr int plane_threads() { return (sizeof(T) == 4 && N2 * N3 >= 16384) ? 512 : 1024; }
 
+// The block's work for plane `blk` (= field * N1 + i1); sm is the LDS plane.
 template <typename T, int N2, int N3, bool MASK, bool TAUPP>
-__global__ void __launch_bounds__((plane_threads<T, N2, N3>()))
-k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
-             const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
-             const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
+__device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, double2* __restrict__ xi,
+                                           const uint8_t* __restrict__ mask, const double* __restrict__ taupp,
+                                           const StepArgs& a, const cx<T>* __restrict__ tw2,
+                                           const cx<T>* __restrict__ tw3, double* __restrict__ partials,
+                                           const int* __restrict__ step_ctr, long blk, cx<T>* sm) {
   constexpr int LD = N3 + 1, PLANE = N2 * N3, PLANE_THREADS = plane_threads<T, N2, N3>();
   constexpr int TPP2 = make_plan(N2).tpp, TPP3 = make_plan(N3).tpp;
   constexpr int PPI2 = PLANE_THREADS / TPP2, PPI3 = PLANE_THREADS / TPP3;
   constexpr int G2 = (N3 + PPI2 - 1) / PPI2, G3 = (N2 + PPI3 - 1) / PPI3;  // N3 column pencils, N2 row pencils
-  extern __shared__ unsigned char lds_raw[];
-  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   __shared__ double red[PLANE_THREADS / 64];
   const long P = a.P;
   const int n1 = (int)(P / PLANE);
-  const int f = blockIdx.x / n1, i1 = blockIdx.x - f * n1;
+  const int f = (int)(blk / n1), i1 = (int)(blk - (long)f * n1);
   const long off = (long)f * P + (long)i1 * PLANE;
   const ColPencils<T> gcol{W + off, N3};
   const LdsColPencils<T> lcol{sm, LD};
@@ -642,6 +650,16 @@ k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __r
   if (t == 0) partials[(long)f * n1 + i1] = sum;
 }
 
+template <typename T, int N2, int N3, bool MASK, bool TAUPP>
+__global__ void __launch_bounds__((plane_threads<T, N2, N3>()))
+k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
+             const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
+             const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
+  extern __shared__ unsigned char lds_raw[];
+  plane_body<T, N2, N3, MASK, TAUPP>(W, xi, mask, taupp, a, tw2, tw3, partials, step_ctr, blockIdx.x,
+                                     reinterpret_cast<cx<T>*>(lds_raw));
+}
+
 // K3: one block owns `tile` consecutive (i2,i3) columns of every field.
 // Forward transform along N1 (last pass of G.c:4599), contraction with B(k)
 // at every k of the tile (G.c:4609-4629), inverse transform along N1 (first
@@ -654,18 +672,18 @@ template <typename T, int N> constexpr int c3_max_pencils() {
   return (sizeof(T) == 8 ? 1 : 2) * (C3_THREADS / make_plan(N).tpp);
 }
 
+// The block's work for column tile `bt` of instance `inst`; sm holds the
+// S * tile pencils.
 template <typename T, int N, int SM>
-__global__ void __launch_bounds__(C3_THREADS)
-k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int tile, int quirk,
-               const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void col_contract_body(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P,
+                                                  int tile, int quirk, const cx<T>* __restrict__ tw, long bt, int inst,
+                                                  cx<T>* sm) {
   constexpr int TPP = make_plan(N).tpp, LD = pitch<N>(), PPI = C3_THREADS / TPP;
   constexpr int G = (c3_max_pencils<T, N>() + PPI - 1) / PPI;
-  extern __shared__ unsigned char lds_raw[];
-  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   const int S = ec->ns;
-  W += (long)blockIdx.y * S * P;  // instance
+  W += (long)inst * S * P;
   const long inner = P / N;
-  const long m0 = xcd_tile(blockIdx.x, gridDim.x) * tile;
+  const long m0 = bt * tile;
   const int npen = S * tile;
   const int per_field = N * tile;
   for (int e = threadIdx.x; e < S * per_field; e += C3_THREADS) {
@@ -690,6 +708,16 @@ k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
     const int fld = e / per_field, r = e - fld * per_field, n = r / tile, p = r - n * tile;
     W[(long)fld * P + (long)n * inner + m0 + p] = sm[(fld * tile + p) * LD + n];
   }
+}
+
+// grid = (column tiles, instances)
+template <typename T, int N, int SM>
+__global__ void __launch_bounds__(C3_THREADS)
+k_col_contract(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int tile, int quirk,
+               const cx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char lds_raw[];
+  col_contract_body<T, N, SM>(W, ec, P, tile, quirk, tw, xcd_tile(blockIdx.x, gridDim.x), blockIdx.y,
+                              reinterpret_cast<cx<T>*>(lds_raw));
 }
 
 // Stored B(k): upper triangle in the working precision, built in FP64.
@@ -810,6 +838,106 @@ k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, con
   for (int c = 0; c < 18; c++) {
     const double s = block_sum(c < 9 ? e[c] : sg[c - 9], red);
     if (threadIdx.x == 0) partials[(long)c * nblk + blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------- persistent ensemble step
+// One cooperative launch runs a whole GL step of every class of an ensemble
+// (of one precision): phase 1 takes plane items (plane_body) of all classes
+// from a work counter, a grid barrier follows, and phase 2 takes the gamma
+// finish of each class and then the fused column items (col_contract_body).
+// The items run exactly the code of the plane / fused column / finish
+// kernels, so a step gives the same bits as those kernels launched per
+// class.  Supported classes: cubic grids N in {32, 48, 64, 96}, S <= 4,
+// recomputed B, no frozen points, uniform resolved shear.
+constexpr int ENS_THREADS = 1024, ENS_MAX_CLASSES = 8;
+struct EnsClassDev {
+  void* W;                 // cx<T> [B*S][P]
+  double2* xi;
+  const ElasticConsts* ec;
+  const void* tw[3];       // cx<T> twiddles of the N1, N2, N3 axes
+  StepArgs sa;
+  double* partials;
+  double* hist;
+  int* step_ctr;
+  long P;
+  int n, F, B, S, c3_tile;
+  long plane_first, col_first;  // first item of the class in phase 1 / phase 2
+};
+struct EnsDev {
+  int* q;                  // work counters of the two phases (zero between launches)
+  int nc;
+  long n_plane, n_col;     // items per phase; phase 2 starts with the nc finish items
+  EnsClassDev cls[ENS_MAX_CLASSES];
+};
+
+// (Bodies are not inlined into the persistent kernel, so its registers are
+// those of the largest body rather than of all of them together.)
+template <typename T, int N>
+__device__ __noinline__ void ens_plane(const EnsClassDev* __restrict__ c, long local, cx<T>* sm) {
+  plane_body<T, N, N, false, false>(reinterpret_cast<cx<T>*>(c->W), c->xi, nullptr, nullptr, c->sa,
+                                    reinterpret_cast<const cx<T>*>(c->tw[1]),
+                                    reinterpret_cast<const cx<T>*>(c->tw[2]), c->partials, c->step_ctr, local, sm);
+}
+template <typename T, int N>
+__device__ __noinline__ void ens_cols(const EnsClassDev* __restrict__ c, long local, cx<T>* sm) {
+  const long tiles = (long)N * N / c->c3_tile;
+  col_contract_body<T, N, 4>(reinterpret_cast<cx<T>*>(c->W), c->ec, c->P, c->c3_tile, 0,
+                             reinterpret_cast<const cx<T>*>(c->tw[0]), local % tiles, (int)(local / tiles), sm);
+}
+__device__ __noinline__ void ens_finish(const EnsClassDev* __restrict__ c, double* fsum) {
+  finish_body(c->partials, c->F, c->n, c->S, c->hist, c->step_ctr, fsum);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ENS_THREADS) k_ens_step(const EnsDev* __restrict__ d) {
+  extern __shared__ unsigned char lds_raw[];
+  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  __shared__ int slot;
+  // phase 2's counter is idle until the barrier: clear it now
+  if (blockIdx.x == 0 && threadIdx.x == 0) d->q[1] = 0;
+  for (;;) {
+    __syncthreads();  // the previous item is done with LDS and `slot`
+    if (threadIdx.x == 0) slot = atomicAdd(&d->q[0], 1);
+    __syncthreads();
+    const long it = slot;
+    if (it >= d->n_plane) break;
+    int k = 0;
+    while (k + 1 < d->nc && it >= d->cls[k + 1].plane_first) k++;
+    const EnsClassDev* c = &d->cls[k];
+    const long local = it - c->plane_first;
+    switch (c->n) {
+      case 32: ens_plane<T, 32>(c, local, sm); break;
+      case 48: ens_plane<T, 48>(c, local, sm); break;
+      case 64: ens_plane<T, 64>(c, local, sm); break;
+      case 96: ens_plane<T, 96>(c, local, sm); break;
+      default: break;
+    }
+  }
+  cooperative_groups::this_grid().sync();
+  // every block has drawn its last phase-1 item: clear that counter for the next launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) d->q[0] = 0;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) slot = atomicAdd(&d->q[1], 1);
+    __syncthreads();
+    const long it = slot;
+    if (it >= d->n_col) break;
+    if (it < d->nc) {
+      ens_finish(&d->cls[it], reinterpret_cast<double*>(lds_raw));
+      continue;
+    }
+    int k = 0;
+    while (k + 1 < d->nc && it >= d->cls[k + 1].col_first) k++;
+    const EnsClassDev* c = &d->cls[k];
+    const long local = it - c->col_first;
+    switch (c->n) {
+      case 32: ens_cols<T, 32>(c, local, sm); break;
+      case 48: ens_cols<T, 48>(c, local, sm); break;
+      case 64: ens_cols<T, 64>(c, local, sm); break;
+      case 96: ens_cols<T, 96>(c, local, sm); break;
+      default: break;
+    }
   }
 }
 

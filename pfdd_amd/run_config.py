@@ -5,9 +5,10 @@ mixed-precision error against the FP64 run of the same instance.
     python -m pfdd_amd.run_config c4 [--steps N] [--every 50] [--json out.json]
     python -m pfdd_amd.run_config c3 [--steps N] [--count 512] [--json out.json]
 
-c2: 256^3 x 12, 64 seeded loops: mixed and FP64 GL runs of N steps (default
-    2000); reports GL steps/s, grid-points x systems / s, and the errors of the
-    mixed run in xi (max abs), slipped area (sum_a gamma_a P) and mean stress.
+c2: 256^3 x 12, 64 seeded loops: FP64 and mixed (stored and recomputed B) GL
+    runs of N steps (default 2000); reports GL steps/s, grid-points x systems
+    / s, and the errors of the mixed runs in xi (max abs), slipped area
+    (sum_a gamma_a P) and mean stress.
 c4: 512^3 x 3 with the applied-shear ramp 0 -> 0.03 mu: mixed (recompute B)
     and FP64 runs with <sigma>, <eps> sampled every `every` steps (the
     stress-strain curve), plus mixed with stored B over a shorter window for
@@ -20,7 +21,8 @@ import time
 
 import numpy as np
 
-from . import B_RECOMPUTE, B_STORED, FP64, MIXED, Context, Ensemble, describe_plan
+from . import (B_RECOMPUTE, B_STORED, ENS_PERSISTENT, ENS_PHASED, FP64, MIXED, Context, Ensemble, PfddError,
+               describe_plan)
 from . import configs as C
 from . import hip
 
@@ -113,8 +115,29 @@ def run_ensemble(w, steps, precision, tune_steps=8):
     t1 = time.time()
     plans, tuned_ms = e.autotune(tune_steps)
     tune_s = time.time() - t1
+    mode, t_ph, t_pe = e.mode()
     e.begin_outer(True)
     ms = _timed_ens(e, steps)
+    # the mode the autotuner did not keep, over 64 steps from the initial state
+    other = {}
+    try:
+        xs = [e.download_xi(i) for i in range(w.count)]
+        for i in range(w.count):
+            e.upload_xi(i, w.member(i).xi)
+        if mode == ENS_PHASED:
+            e.set_mode(ENS_PERSISTENT)
+        else:
+            e.set_mode(ENS_PHASED)
+            for k in range(ncls):
+                e.set_plan(k, int(plans[k]))
+        e.begin_outer(True)
+        e.gl_steps(8, record=False)
+        other = {"mode": "persistent" if mode == ENS_PHASED else "phased", "ms_per_step": _timed_ens(e, 64) / 64}
+        for i in range(w.count):  # leave the final state of the measured run in place
+            e.upload_xi(i, xs[i])
+        del xs
+    except PfddError as err:
+        other = {"error": str(err)}
     classes = []
     for k in range(ncls):
         members = np.nonzero(cls == k)[0]
@@ -125,7 +148,9 @@ def run_ensemble(w, steps, precision, tune_steps=8):
     out = {"steps": steps, "ms": ms, "steps_per_s": steps / (ms / 1e3),
            "gridpoints_x_systems_per_s": pts * steps / (ms / 1e3),
            "auto_plan_ms_per_step": auto_ms, "auto_plan_gridpoints_x_systems_per_s": pts / (auto_ms / 1e3),
-           "classes": classes, "setup_s": setup_s, "autotune_s": tune_s, "device_bytes": e.device_bytes}
+           "classes": classes, "setup_s": setup_s, "autotune_s": tune_s, "device_bytes": e.device_bytes,
+           "mode": "persistent" if mode == ENS_PERSISTENT else "phased",
+           "autotune_ms_per_step": {"phased": t_ph, "persistent": t_pe}, "other_mode": other}
     return out, e
 
 
@@ -152,7 +177,16 @@ def main():
         r64, x64, ga64, e64, s64 = run_plain(w, steps, FP64, B_RECOMPUTE)
         print(f"[run_config] fp64 {r64['steps_per_s']:.2f} steps/s", flush=True)
         r32, x32, ga32, e32, s32 = run_plain(w, steps, MIXED, B_STORED)
-        print(f"[run_config] mixed {r32['steps_per_s']:.2f} steps/s", flush=True)
+        print(f"[run_config] mixed stored B {r32['steps_per_s']:.2f} steps/s", flush=True)
+        rr, xr, gar, er, sr = run_plain(w, steps, MIXED, B_RECOMPUTE)
+        print(f"[run_config] mixed recomputed B {rr['steps_per_s']:.2f} steps/s", flush=True)
+        res["mixed_recompute_B"] = dict(rr, speedup=rr["steps_per_s"] / r64["steps_per_s"],
+                                        xi_max_abs_err=float(np.abs(xr - x64).max()),
+                                        slipped_area_rel_err=float(abs(gar[-1].sum() * w.P - ga64[-1].sum() * w.P)
+                                                                   / abs(ga64[-1].sum() * w.P)),
+                                        mean_stress_max_abs_err=float(np.abs(sr - s64).max()),
+                                        step_roofline_frac=step_bytes(w.ns, w.P, False, False) * rr["steps_per_s"] / 8e12)
+        del xr
         area64, area32 = ga64[-1].sum() * w.P, ga32[-1].sum() * w.P
         res.update({"shape": w.shape, "ns": w.ns, "steps": steps, "fp64": r64, "mixed_stored_B": r32,
                     "speedup": r32["steps_per_s"] / r64["steps_per_s"],

@@ -227,3 +227,82 @@ def test_time_step_kernels_follows_the_plan_and_advances_state():
             x = c.download_xi()
         xs, _ = single_run(w, m, 5, pa.FP64, plan=plan)
         assert np.array_equal(x, xs), plan
+
+
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+def test_persistent_step_equals_phased(prec):
+    """The persistent cooperative step (one launch per step) runs the same
+    item code as the per-class kernels: an ensemble stepped both ways, with
+    the plans the persistent step uses, agrees bit for bit (xi and gamma)."""
+    w = small_ensemble(count=5, sizes=(32, 48, 64), first=40)
+    members = [w.member(i) for i in range(w.count)]
+    plan = pa.PLAN_PLANE | pa.PLAN_FUSED_COLS
+    say(f"persistent vs phased prec={prec} shapes={[m.shape[0] for m in members]}")
+    out = []
+    for mode in (pa.ENS_PHASED, pa.ENS_PERSISTENT):
+        with pa.Ensemble(w.params_list(prec, plan=plan)) as e:
+            for m in members:
+                e.set_tau(m.index, m.tau)
+                e.upload_xi(m.index, m.xi)
+            e.set_mode(mode)
+            assert e.mode()[0] == mode
+            e.begin_outer(True)
+            g = e.gl_steps(21)
+            out.append((g, [e.download_xi(i) for i in range(w.count)]))
+    (g0, x0), (g1, x1) = out
+    assert np.array_equal(g0, g1)
+    for i in range(w.count):
+        assert np.array_equal(x0[i], x1[i]), i
+
+
+def test_persistent_step_vs_oracle_and_limits():
+    """Persistent mode against the oracle (FP64, 30 steps, <= 1e-10); grids
+    and inputs it does not cover are refused."""
+    w = small_ensemble(count=3, sizes=(32, 48), first=60)
+    members = [w.member(i) for i in range(w.count)]
+    say("persistent vs oracle")
+    with pa.Ensemble(w.params_list(pa.FP64)) as e:
+        for m in members:
+            e.set_tau(m.index, m.tau)
+            e.upload_xi(m.index, m.xi)
+        e.set_mode(pa.ENS_PERSISTENT)
+        e.begin_outer(True)
+        e.gl_steps(30)
+        got = [e.download_xi(i) for i in range(w.count)]
+        # a per-point shear is outside the persistent step
+        e.set_tau(0, np.full((members[0].P, 3), 0.01), per_point=True)
+        with pytest.raises(pa.PfddError) as ei:
+            e.gl_steps(1)
+        assert ei.value.rc == pa.E_UNSUPPORTED
+    for m in members:
+        xo, _ = oracle_run(w, m, 30)
+        assert np.abs(got[m.index] - xo).max() <= 1e-10, m.index
+    w16 = small_ensemble(count=2, sizes=(16,))
+    with pa.Ensemble(w16.params_list(pa.FP64)) as e:
+        with pytest.raises(pa.PfddError) as ei:
+            e.set_mode(pa.ENS_PERSISTENT)
+        assert ei.value.rc == pa.E_UNSUPPORTED
+        assert e.mode()[0] == pa.ENS_PHASED
+
+
+def test_ensemble_autotune_chooses_a_mode():
+    """The ensemble autotuner times both modes, keeps the faster, and the run
+    after it matches single contexts to rounding."""
+    w = small_ensemble(count=3, sizes=(32, 48), first=80)
+    members = [w.member(i) for i in range(w.count)]
+    with pa.Ensemble(w.params_list(pa.MIXED)) as e:
+        for m in members:
+            e.set_tau(m.index, m.tau)
+            e.upload_xi(m.index, m.xi)
+        e.autotune(8)
+        mode, t_ph, t_pe = e.mode()
+        say(f"ensemble mode {mode}: phased {t_ph:.4f} ms, persistent {t_pe:.4f} ms")
+        assert t_ph > 0 and t_pe > 0 and mode == (pa.ENS_PERSISTENT if t_pe < t_ph else pa.ENS_PHASED)
+        for m in members:
+            assert np.array_equal(e.download_xi(m.index), m.xi)
+        e.begin_outer(True)
+        e.gl_steps(16)
+        got = [e.download_xi(i) for i in range(w.count)]
+    for m in members:
+        xs, _ = single_run(w, m, 16, pa.MIXED)
+        assert np.abs(got[m.index] - xs).max() <= 2e-5, m.index
