@@ -132,8 +132,8 @@ void launch_axis_n(cx<T>* data, long outer, long inner, int sign, const cx<T>* t
     int tile = 1;
     while (tile * 2 <= TILE && inner % (tile * 2) == 0) tile *= 2;
     const long nb = outer * (inner / tile);
-    hipLaunchKernelGGL((k_fft_cols<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, inner, tile,
-                       (T)sign, tw);
+    hipLaunchKernelGGL((k_fft_cols<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), (cols_lds_bytes<T, N, RL>()), st,
+                       data, inner, tile, (T)sign, tw);
   }
 }
 
@@ -147,6 +147,32 @@ void launch_axis_r(cx<T>* data, long outer, long inner, int sign, const cx<T>* t
     }
   }
   launch_axis_n<T, N, 8>(data, outer, inner, sign, tw, st);
+}
+
+// The column kernels of length N may take more than 64 KB of dynamic LDS;
+// the attribute is set once per process for every instantiation (in
+// ctx_create, outside any graph capture).
+template <typename T, int N>
+int cols_attr_n() {
+  HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols<T, N, 8>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds_bytes<T, N, 8>()));
+  if constexpr (has_r4_variant(N))
+    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols<T, N, 4>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds_bytes<T, N, 4>()));
+  return PFDD_OK;
+}
+
+int cols_attr(int n) {
+  int rc = PFDD_OK;
+  switch (n) {
+#define PFDD_LEN(NN) case NN: if (!(rc = cols_attr_n<double, NN>())) rc = cols_attr_n<float, NN>(); break;
+    PFDD_LEN(2) PFDD_LEN(3) PFDD_LEN(4) PFDD_LEN(6) PFDD_LEN(8) PFDD_LEN(12) PFDD_LEN(16) PFDD_LEN(24)
+    PFDD_LEN(32) PFDD_LEN(48) PFDD_LEN(64) PFDD_LEN(96) PFDD_LEN(128) PFDD_LEN(192) PFDD_LEN(256)
+    PFDD_LEN(384) PFDD_LEN(512)
+#undef PFDD_LEN
+    default: return PFDD_E_RADIX;
+  }
+  return rc;
 }
 
 // Transform along one axis of an [outer][n][inner] complex array.
@@ -761,9 +787,11 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     if (hipMemsetAsync(c->xi, 0, sizeof(double2) * F * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if (hipMemsetAsync(c->W, 0, csz * F * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if ((rc = push_tau(c))) goto fail;
-    // twiddle tables, one per axis
-    for (int i = 0; i < 3; i++)
+    // twiddle tables, one per axis; LDS attributes of the axis kernels
+    for (int i = 0; i < 3; i++) {
       if ((rc = make_twiddles(c->tw[i], ns3[i], c->bytes, c->stream))) goto fail;
+      if ((rc = cols_attr(ns3[i]))) goto fail;
+    }
     // elastic constants (Bmatrix G.c:340-358; s_a = C:e_a etc.)
     ElasticConsts h{};
     h.mu = p->C44;
@@ -1341,8 +1369,10 @@ extern "C" int pfdd_time_step_kernels(pfdd_ctx* c, int nsteps, float* ms, int* l
   std::vector<hipEvent_t> pool((size_t)nsteps * per_step_max + 1, nullptr);
   std::vector<std::pair<int, hipEvent_t>> taps;
   int rc = PFDD_OK;
+  // timing-only events: no system-scope fence (cache writeback) at each record,
+  // which would otherwise add to the time of the launch before it
   for (auto& e : pool)
-    if (hipEventCreate(&e) != hipSuccess) rc = PFDD_E_HIP;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) rc = PFDD_E_HIP;
   if (rc == PFDD_OK) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
     taps.reserve(pool.size());

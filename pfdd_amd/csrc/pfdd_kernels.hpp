@@ -131,16 +131,20 @@ template <typename T, int N, int RL = 8> constexpr int tile_rows() {
   return t;
 }
 // Columns per block of the strided-axis kernel: at least 256 bytes of each
-// row are read together (16 FP64 or 32 FP32 columns), within the LDS and
-// 1024-thread limits.
+// row are read together (16 FP64 or 32 FP32 columns), within 1024 threads
+// and 80 KB of (dynamic) LDS, so that two blocks still fit on a CU.
+constexpr int COLS_LDS_MAX = 80 * 1024;
 template <typename T, int N, int RL = 8> constexpr int tile_cols() {
   constexpr int tpp = make_plan(N, RL).tpp;
   int t = pow2_floor(256 / tpp > 0 ? 256 / tpp : 1);
   if (t < 256 / (int)sizeof(cx<T>)) t = 256 / (int)sizeof(cx<T>);
-  const int lim = 65536 / (pitch<N>() * (int)sizeof(cx<T>));
+  const int lim = COLS_LDS_MAX / (pitch<N>() * (int)sizeof(cx<T>));
   while (t > lim && t > 1) t /= 2;
   while (t * tpp > 1024 && t > 1) t /= 2;
   return t;
+}
+template <typename T, int N, int RL = 8> constexpr size_t cols_lds_bytes() {
+  return (size_t)tile_cols<T, N, RL>() * pitch<N>() * sizeof(cx<T>);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -176,7 +180,8 @@ template <typename T, int N, int RL = 8>
 __global__ void __launch_bounds__((tile_cols<T, N, RL>() * make_plan(N, RL).tpp))
 k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* __restrict__ tw) {
   constexpr int TILE = tile_cols<T, N, RL>(), LD = pitch<N>();
-  __shared__ cx<T> sm[TILE * LD];
+  extern __shared__ unsigned char lds_raw[];  // cols_lds_bytes<T, N, RL>()
+  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   const long tiles_per_outer = inner / tile;
   const long bt = xcd_tile(blockIdx.x, gridDim.x);
   const long o = bt / tiles_per_outer;

@@ -50,7 +50,8 @@ def run_oracle(w, steps, quirk=True, xi_bc=None, tau=None, tau_pp=False):
 
 # ---------------------------------------------------------------- FFT (row a1)
 FFT_SHAPES = [(32, 2, 32), (16, 16, 16), (8, 12, 6), (48, 48, 48), (96, 32, 16), (128, 128, 128),
-              (3, 4, 2), (256, 64, 4), (512, 8, 2), (24, 384, 2), (64, 192, 12)]
+              (3, 4, 2), (256, 64, 4), (512, 8, 2), (24, 384, 2), (64, 192, 12), (8, 512, 64), (512, 4, 64),
+              (16, 256, 128)]
 
 
 @pytest.mark.parametrize("shape", FFT_SHAPES)
