@@ -5,7 +5,8 @@
 A step is one GL iteration (update, forward 3D FFT, B(k) contraction, inverse
 3D FFT) over the whole 128^3 x 12 instance, inputs resident in HBM.  Prints one
 JSON line.  value = FP64 GL steps/s; the mixed-precision path (FP32 FFT and
-contraction, FP64 master xi) is timed beside it with its error against FP64.
+contraction, compensated master xi updated in FP64) is timed beside it with
+its error against FP64.
 """
 import argparse
 import json
@@ -32,17 +33,18 @@ STAGES = [("k_update", pa.STAGE_UPDATE, 1), ("k_fft_rows(N3)", pa.STAGE_FFT_AXIS
 def stage_bytes(stage, S, P, ntri, fp64, stored):
     """Algorithmic bytes of one launch: each input read once, each output written once."""
     c = 16 if fp64 else 8
+    m = 16 if fp64 else 12                        # master xi: double2, or the compensated FP32 form
     if stage == pa.STAGE_FINISH:
         return 0
     r = 8 if fp64 else 4
     if stage == pa.STAGE_UPDATE:
-        return S * P * (c + 16 + 16 + c)          # read W, read xi, write xi, write W
+        return S * P * (c + m + m + c)            # read W, read xi, write xi, write W
     if stage in (pa.STAGE_FFT_AXIS3, pa.STAGE_FFT_AXIS2, pa.STAGE_FFT_AXIS1):
         return S * P * 2 * c
     if stage == pa.STAGE_CONTRACT:
         return S * P * 2 * c + (ntri * P * r if stored else 0)
     if stage == pa.STAGE_COPY:
-        return S * P * (16 + c)
+        return S * P * (m + c)
     raise ValueError(stage)
 
 
@@ -79,10 +81,10 @@ def measured_traffic(name):
 
 def step_alg_bytes(S, P, fp64, stored):
     """Stage-minimum bytes of one GL step (C2C, FP64: (144 S [+4 S(S+1)]) P;
-    mixed: (96 S [+2 S(S+1)]) P)."""
+    mixed with the 12-byte compensated master: (88 S [+2 S(S+1)]) P)."""
     if fp64:
         return (144 * S + (4 * S * (S + 1) if stored else 0)) * P
-    return (96 * S + (2 * S * (S + 1) if stored else 0)) * P
+    return (88 * S + (2 * S * (S + 1) if stored else 0)) * P
 
 
 def timed_steps(ctx, steps, warmup):

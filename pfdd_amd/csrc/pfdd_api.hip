@@ -54,7 +54,8 @@ struct pfdd_ctx {
   hipStream_t stream = nullptr;
   int owns_stream = 1;       // 0: the stream belongs to an ensemble
   // device buffers
-  double2* xi = nullptr;
+  void* xi = nullptr;         // the master: Master<T> over F*P points (16 or 12 bytes each)
+  size_t xi_bytes = 0;
   void* W = nullptr;         // cx<T> [S][P]
   void* Bsym = nullptr;      // T [ntri][P]
   uint8_t* mask = nullptr;
@@ -115,6 +116,8 @@ void tap(pfdd_ctx* c, int stage) {
 template <typename T> const cx<T>* tw_of(const Twiddles& t);
 template <> const double2* tw_of<double>(const Twiddles& t) { return t.d; }
 template <> const float2* tw_of<float>(const Twiddles& t) { return t.f; }
+
+template <typename T> Master<T> xm(const pfdd_ctx* c) { return make_master<T>(c->xi, (long)c->F * c->P); }
 
 // radix limit of the standalone passes along axis ax (0: N1, 1: N2, 2: N3)
 int rl_of(const pfdd_ctx* c, int ax) { return (c->r4 >> ax) & 1 ? 4 : 8; }
@@ -224,7 +227,7 @@ int update(pfdd_ctx* c, bool finish = true) {
   cx<T>* W = (cx<T>*)c->W;
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
 #define PFDD_UP(MM, TT)                                                                                  \
-  hipLaunchKernelGGL((k_update<T, MM, TT>), grid, dim3(UPD_THREADS), 0, c->stream, c->xi, W, c->mask, c->taupp, \
+  hipLaunchKernelGGL((k_update<T, MM, TT>), grid, dim3(UPD_THREADS), 0, c->stream, xm<T>(c), W, c->mask, c->taupp, \
                      c->sa, c->partials, c->step_ctr)
   if (m && tp) PFDD_UP(true, true);
   else if (m) PFDD_UP(true, false);
@@ -270,7 +273,7 @@ void k1_launch_n(pfdd_ctx* c) {
   constexpr int NT = tile_rows<T, N>() * make_plan(N).tpp;
   const cx<T>* tw = tw_of<T>(c->tw[2]);
 #define PFDD_K1(MM, TT)                                                                                   \
-  hipLaunchKernelGGL((k_row_step<T, N, MM, TT>), dim3((unsigned)nb), dim3(NT), 0, c->stream, W, c->xi, c->mask, \
+  hipLaunchKernelGGL((k_row_step<T, N, MM, TT>), dim3((unsigned)nb), dim3(NT), 0, c->stream, W, xm<T>(c), c->mask, \
                      c->taupp, c->sa, c->row_tile, tw, c->partials, c->step_ctr)
   const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
   if (m && tp) PFDD_K1(true, true);
@@ -348,7 +351,7 @@ int plane_launch_n(pfdd_ctx* c, bool attr_only) {
     HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_step<T, N, N, MM, TT>),                     \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                           \
   } else {                                                                                                       \
-    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT>), dim3(nb), dim3(plane_threads<T, N, N>()), lds, c->stream, W, c->xi,  \
+    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT>), dim3(nb), dim3(plane_threads<T, N, N>()), lds, c->stream, W, xm<T>(c),  \
                        c->mask, c->taupp, c->sa, tw2, tw3, c->partials, c->step_ctr);                           \
   }
   if (attr_only) {
@@ -457,7 +460,7 @@ template <typename T>
 int prologue(pfdd_ctx* c, int quirk) {
   int rc;
   const long n = (long)c->F * c->P;
-  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, (cx<T>*)c->W, n);
+  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c), (cx<T>*)c->W, n);
   if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
   if (c->plane) {  // the next step's plane kernel finishes the N2 and N3 inverse passes
     if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD,
@@ -538,7 +541,7 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
   // 1. <xi_a> (avestrain G.c:670-686)
   {
     dim3 grid((unsigned)c->upd_nblk, (unsigned)S);
-    hipLaunchKernelGGL(k_xi_sums, grid, dim3(UPD_THREADS), 0, c->stream, c->xi, P, c->upd_nblk, c->partials);
+    hipLaunchKernelGGL((k_xi_sums<T>), grid, dim3(UPD_THREADS), 0, c->stream, xm<T>(c), P, c->upd_nblk, c->partials);
     hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, S, c->upd_nblk, S,
                        c->small, (int*)nullptr);
     HIPCK(hipPeekAtLastError());
@@ -556,7 +559,7 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
   // 2. spectrum of xi, strain spectrum, 6 inverse transforms (the work array
   //    is rebuilt from xi afterwards, so the GL state is left untouched)
   const unsigned nb = (unsigned)((P + 255) / 256);
-  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((S * P + 255) / 256)), dim3(256), 0, c->stream, c->xi, W, S * P);
+  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((S * P + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c), W, S * P);
   if ((rc = fft3d<T>(c, W, S, PFDD_FORWARD))) return rc;
 #define PFDD_SS(SM) hipLaunchKernelGGL((k_strain_spectrum<T, SM>), dim3(nb), dim3(256), 0, c->stream, W, E, c->ec, P)
   if (S <= 4) PFDD_SS(4); else if (S <= 8) PFDD_SS(8); else PFDD_SS(16);
@@ -567,7 +570,7 @@ int strain_stress_impl(pfdd_ctx* c, const double* avepst_uniform, double* aveps,
   double* ef = fields ? c->efield : nullptr;
   double* sf = fields ? c->sfield : nullptr;
 #define PFDD_SR(SM)                                                                                       \
-  hipLaunchKernelGGL((k_strain_stress<T, SM>), dim3(nb), dim3(256), 0, c->stream, E, c->xi, c->ec, c->avepst_d, P, \
+  hipLaunchKernelGGL((k_strain_stress<T, SM>), dim3(nb), dim3(256), 0, c->stream, E, xm<T>(c), c->ec, c->avepst_d, P, \
                      c->partials, (int)nb, ef, sf)
   if (S <= 4) PFDD_SR(4); else if (S <= 8) PFDD_SR(8); else PFDD_SR(16);
 #undef PFDD_SR
@@ -774,7 +777,8 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     // block sums of every plan (the row kernel's tile may go down to one row)
     c->partials_len = (size_t)std::max<long>(std::max<long>((long)F * c->upd_nblk, 18 * strain_nblk),
                                              std::max<long>((long)F * p->n1 * p->n2, (long)F * p->n1));
-    if (!(c->xi = (double2*)dev_alloc<char>(sizeof(double2) * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
+    c->xi_bytes = (c->fp64 ? master_bytes<double>() : master_bytes<float>()) * F * P;
+    if (!(c->xi = dev_alloc<char>(c->xi_bytes, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->W = dev_alloc<char>(csz * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->hist = (double*)dev_alloc<char>(sizeof(double) * HIST_CAP * c->B * (S + 1), c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -784,7 +788,7 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     if (!(c->ec = (ElasticConsts*)dev_alloc<char>(sizeof(ElasticConsts), c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->taud = (double2*)dev_alloc<char>(sizeof(double2) * F, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (hipMemsetAsync(c->step_ctr, 0, sizeof(int) * 4, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
-    if (hipMemsetAsync(c->xi, 0, sizeof(double2) * F * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
+    if (hipMemsetAsync(c->xi, 0, c->xi_bytes, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if (hipMemsetAsync(c->W, 0, csz * F * P, c->stream) != hipSuccess) FAIL(PFDD_E_HIP);
     if ((rc = push_tau(c))) goto fail;
     // twiddle tables, one per axis; LDS attributes of the axis kernels
@@ -952,7 +956,22 @@ int upload_inst(pfdd_ctx* c, int b, const double* xi, const double* xi_bc, int n
     }
   }
   HIPCK(hipStreamSynchronize(c->stream));
-  HIPCK(hipMemcpyAsync(c->xi + off, xi, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+  if (c->fp64) {
+    HIPCK(hipMemcpyAsync((double2*)c->xi + off, xi, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+  } else {  // split into the compensated form of Master<float>
+    std::vector<float2> r(n);
+    std::vector<float> im(n);
+    for (size_t i = 0; i < n; i++) {
+      const double re = xi[2 * i];
+      const float hi = (float)re;
+      r[i] = make_float2(hi, (float)(re - (double)hi));
+      im[i] = (float)xi[2 * i + 1];
+    }
+    const Master<float> m = xm<float>(c);
+    HIPCK(hipMemcpyAsync(m.r + off, r.data(), sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(m.im + off, im.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
   c->inst_frozen[b] = frozen ? 1 : 0;
   bool any = false;
   for (uint8_t v : c->inst_frozen) any = any || v;
@@ -986,8 +1005,22 @@ int download_inst(pfdd_ctx* c, int b, double* xi, int nsv) {
   const long P = c->P;
   const size_t n = (size_t)c->S * P;
   HIPCK(hipStreamSynchronize(c->stream));
-  HIPCK(hipMemcpyAsync(xi, c->xi + (size_t)b * n, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
+  if (c->fp64) {
+    HIPCK(hipMemcpyAsync(xi, (double2*)c->xi + (size_t)b * n, sizeof(double2) * n, hipMemcpyDeviceToHost,
+                         c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float2> r(n);
+    std::vector<float> im(n);
+    const Master<float> m = xm<float>(c);
+    HIPCK(hipMemcpyAsync(r.data(), m.r + (size_t)b * n, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(im.data(), m.im + (size_t)b * n, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) {
+      xi[2 * i] = (double)r[i].x + (double)r[i].y;
+      xi[2 * i + 1] = (double)im[i];
+    }
+  }
   if (nsv > c->S) memset(xi + 2 * n, 0, sizeof(double) * 2 * (size_t)(nsv - c->S) * P);
   return PFDD_OK;
 }
@@ -1117,7 +1150,7 @@ std::vector<int> plan_candidates(pfdd_ctx* c) {
 int autotune_ctx(pfdd_ctx* c, int steps, int* best_plan, float* best_ms) {
   if (!c->has_xi) return PFDD_E_STATE;
   if (steps < 1) return PFDD_E_SHAPE;
-  const size_t xbytes = sizeof(double2) * (size_t)c->F * c->P;
+  const size_t xbytes = c->xi_bytes;
   void* xsave = nullptr;
   int* csave = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1336,7 +1369,7 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
         rc = one_step<T>(c);
         break;
       case PFDD_STAGE_COPY:
-        hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->xi, W, n);
+        hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c), W, n);
         rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
         break;
       default:
@@ -1762,8 +1795,7 @@ int pfdd_ensemble_autotune(pfdd_ens* e, int steps, int* plans, float* ms_per_ste
   auto restore = [&]() -> int {
     for (size_t k = 0; k < e->cls.size(); k++) {
       pfdd_ctx* c = e->cls[k];
-      HIPCK(hipMemcpyAsync(c->xi, xsave[k], sizeof(double2) * (size_t)c->F * c->P, hipMemcpyDeviceToDevice,
-                           e->stream));
+      HIPCK(hipMemcpyAsync(c->xi, xsave[k], c->xi_bytes, hipMemcpyDeviceToDevice, e->stream));
       HIPCK(hipMemcpyAsync(c->step_ctr, csave[k], sizeof(int) * 4, hipMemcpyDeviceToDevice, e->stream));
     }
     HIPCK(hipStreamSynchronize(e->stream));
@@ -1795,7 +1827,7 @@ int pfdd_ensemble_autotune(pfdd_ens* e, int steps, int* plans, float* ms_per_ste
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { rc = PFDD_E_HIP; break; }
     for (size_t k = 0; k < e->cls.size() && rc == PFDD_OK; k++) {
       pfdd_ctx* c = e->cls[k];
-      const size_t xb = sizeof(double2) * (size_t)c->F * c->P;
+      const size_t xb = c->xi_bytes;
       if (hipMalloc(&xsave[k], xb) != hipSuccess || hipMalloc(&csave[k], sizeof(int) * 4) != hipSuccess) {
         rc = PFDD_E_ALLOC;
         break;

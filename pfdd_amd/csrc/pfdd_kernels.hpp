@@ -2,7 +2,9 @@
 //
 // Data layout in HBM (one context of B instances of the same grid, S slip
 // systems each; field f = b * S + a; B = 1 for a single run):
-//   xi    double2 [B*S][N1][N2][N3]   FP64 master order parameter (complex, as G.c:64)
+//   xi    master order parameter [B*S][N1][N2][N3] (complex, as G.c:64):
+//         FP64: double2; mixed: Re as float2 (hi, lo) followed by Im as float
+//         (struct Master)
 //   W     cx<T>   [B*S][N1][N2][N3]   FFT work array: data / data2 of the reference
 //   Bsym  T       [S(S+1)/2][P]       upper triangle of B(k), stored mode only (shared by the instances)
 //   mask  u8      [B*S][P]            1 = frozen (xi_bc != 0), only when some point is frozen
@@ -20,6 +22,48 @@
 namespace pfdd {
 
 constexpr int MAXS = 16;
+
+// The master order parameter.  FP64 contexts keep xi as double2.  Mixed
+// contexts keep a compensated form, 12 bytes per point instead of 16: Re xi
+// as the unevaluated sum hi + lo of two floats (48 significant bits; the
+// update itself runs in FP64 on hi + lo and splits the result again), and
+// Im xi as a float.  Im xi is the small component that the non-Hermitian
+// B(k) of the Nyquist planes feeds (|Im xi| ~ 1e-3); the FP32 transforms
+// around it carry errors of the same relative size.
+template <typename T> struct Master;
+template <> struct Master<double> {
+  double2* p;
+  __device__ __forceinline__ double2 load(long i) const { return p[i]; }
+  __device__ __forceinline__ void store(long i, double2 v) const { p[i] = v; }
+  __device__ __forceinline__ double re(long i) const { return p[i].x; }
+};
+template <> struct Master<float> {
+  float2* r;  // (hi, lo) of Re xi
+  float* im;
+  __device__ __forceinline__ double2 load(long i) const {
+    const float2 h = r[i];
+    return make_double2((double)h.x + (double)h.y, (double)im[i]);
+  }
+  __device__ __forceinline__ void store(long i, double2 v) const {
+    const float hi = (float)v.x;
+    r[i] = make_float2(hi, (float)(v.x - (double)hi));
+    im[i] = (float)v.y;
+  }
+  __device__ __forceinline__ double re(long i) const {
+    const float2 h = r[i];
+    return (double)h.x + (double)h.y;
+  }
+};
+// Bytes per point of the master, and the master over a base allocation of n points.
+template <typename T> constexpr size_t master_bytes() { return sizeof(T) == 8 ? 16 : 12; }
+template <typename T> __host__ __device__ inline Master<T> make_master(void* base, long n);
+template <> __host__ __device__ inline Master<double> make_master<double>(void* base, long) {
+  return Master<double>{reinterpret_cast<double2*>(base)};
+}
+template <> __host__ __device__ inline Master<float> make_master<float>(void* base, long n) {
+  float2* r = reinterpret_cast<float2*>(base);
+  return Master<float>{r, reinterpret_cast<float*>(r + n)};
+}
 
 // Constants of the elastic kernel, shared by every k-point (read through the
 // scalar cache: every lane of a wave reads the same address).
@@ -223,11 +267,13 @@ __device__ __forceinline__ FieldCoef field_coef(const StepArgs& a, int f, const 
 // The update of one point (G.c:3485-3489): d is the unnormalised inverse
 // transform of data2.
 template <bool TAUPP>
+// inv_n = 1 / P (exact for the power-of-two grids; within an ulp of the
+// reference's division by nsize otherwise).
 __device__ __forceinline__ void gl_point(double2& x, double dre, double dim, const FieldCoef& c, const StepArgs& a,
-                                         const double* __restrict__ taupp, long pt, double nsize) {
+                                         const double* __restrict__ taupp, long pt, double inv_n) {
   const double t = TAUPP ? taupp[c.tbase + pt * a.ns] + c.rate : c.tau;
-  x.x -= c.cd2 * (a.uq2 * dre / nsize + c.asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / c.dsl);
-  x.y -= a.uq2 * c.cd2 * (dim / nsize);
+  x.x -= c.cd2 * (a.uq2 * dre * inv_n + c.asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / c.dsl);
+  x.y -= a.uq2 * c.cd2 * (dim * inv_n);
 }
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
@@ -300,13 +346,13 @@ k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns
 // and the gamma sums.  grid = (nblk, B*S).
 template <typename T, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__(UPD_THREADS)
-k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
+k_update(Master<T> xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
          const double* __restrict__ taupp, StepArgs a, double* __restrict__ partials,
          const int* __restrict__ step_ctr) {
   __shared__ double red[UPD_THREADS / 64];
   const int f = blockIdx.y;
   const long P = a.P;
-  const double nsize = (double)P;
+  const double inv_n = 1.0 / (double)P;
   const long off = (long)f * P;
   const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
   const FieldCoef c = field_coef(a, f, step_ctr);
@@ -315,40 +361,41 @@ k_update(double2* __restrict__ xi, cx<T>* __restrict__ W, const uint8_t* __restr
   for (int e = 0; e < UPD_EPT; e++) {
     const long pt = base + (long)e * UPD_THREADS;
     if (pt < P) {
-      double2 x = xi[off + pt];
+      double2 x = xi.load(off + pt);
       const cx<T> d = W[off + pt];
       if (!MASK || mask[off + pt] == 0) {
-        gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, nsize);
-        xi[off + pt] = x;
+        gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
+        xi.store(off + pt, x);
       }
       W[off + pt] = {(T)x.x, (T)x.y};
-      acc += x.x / nsize;
+      acc += x.x;  // (the block sum is scaled by 1/P once)
     }
   }
-  const double s = block_sum(acc, red);
+  const double s = block_sum(acc, red) * inv_n;
   if (threadIdx.x == 0) partials[(long)f * a.nblk + blockIdx.x] = s;
 }
 
 // Sum of Re xi per field (avestrain's xi means, G.c:670-686).  grid = (nblk, S).
+template <typename T>
 __global__ void __launch_bounds__(UPD_THREADS)
-k_xi_sums(const double2* __restrict__ xi, long P, int nblk, double* __restrict__ partials) {
+k_xi_sums(Master<T> xi, long P, int nblk, double* __restrict__ partials) {
   __shared__ double red[UPD_THREADS / 64];
   const int f = blockIdx.y;
   const long base = (long)blockIdx.x * (UPD_THREADS * UPD_EPT) + threadIdx.x;
   double acc = 0.0;
   for (int e = 0; e < UPD_EPT; e++) {
     const long pt = base + (long)e * UPD_THREADS;
-    if (pt < P) acc += xi[(long)f * P + pt].x;
+    if (pt < P) acc += xi.re((long)f * P + pt);
   }
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0) partials[(long)f * nblk + blockIdx.x] = s;
 }
 
 template <typename T>
-__global__ void k_xi_to_w(const double2* __restrict__ xi, cx<T>* __restrict__ W, long n) {
+__global__ void k_xi_to_w(Master<T> xi, cx<T>* __restrict__ W, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    const double2 x = xi[i];
+    const double2 x = xi.load(i);
     W[i] = {(T)x.x, (T)x.y};
   }
 }
@@ -526,7 +573,7 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
 // G.c:4599).  The block's sum of Re xi / P goes to partials[f][block].
 template <typename T, int N, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
-k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
+k_row_step(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask,
            const double* __restrict__ taupp, StepArgs a, int tile, const cx<T>* __restrict__ tw,
            double* __restrict__ partials, const int* __restrict__ step_ctr) {
   constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), NT = TILE * TPP, LD = pitch<N>();
@@ -540,28 +587,27 @@ k_row_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __res
   const long pt0 = (row0 - (long)f * rows_per_field) * N;
   const long off = (long)f * P + pt0;
   cx<T>* wb = W + off;
-  double2* xb = xi + off;
   const RowPencils<T> g{wb, N};
   const LdsPencils<T> l{sm, LD};
   const int pen = (int)threadIdx.x / TPP, q = (int)threadIdx.x % TPP;
   fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, tw);
-  const double nsize = (double)P;
+  const double inv_n = 1.0 / (double)P;
   const FieldCoef c = field_coef(a, f, step_ctr);
   double acc = 0.0;
   for (int e = threadIdx.x; e < tile * N; e += NT) {
     const int li = (e / N) * LD + (e % N);
     const cx<T> d = sm[li];
-    double2 x = xb[e];
+    double2 x = xi.load(off + e);
     if (!MASK || mask[off + e] == 0) {
-      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt0 + e, nsize);
-      xb[e] = x;
+      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt0 + e, inv_n);
+      xi.store(off + e, x);
     }
     sm[li] = {(T)x.x, (T)x.y};
-    acc += x.x / nsize;
+    acc += x.x;  // (the block sum is scaled by 1/P once)
   }
   __syncthreads();
   fft_pencils<T, N, 1>(l, l, g, pen, 1 << 30, tile, q, (T)-1, tw);
-  const double sum = block_sum(acc, red);
+  const double sum = block_sum(acc, red) * inv_n;
   if (threadIdx.x == 0) partials[(long)f * (rows_per_field / tile) + (pt0 / N) / tile] = sum;
 }
 
@@ -581,7 +627,7 @@ constexpr int plane_threads() { return (sizeof(T) == 4 && N2 * N3 >= 16384) ? 51
 
 // The block's work for plane `blk` (= field * N1 + i1); sm is the LDS plane.
 template <typename T, int N2, int N3, bool MASK, bool TAUPP>
-__device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, double2* __restrict__ xi,
+__device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, Master<T> xi,
                                            const uint8_t* __restrict__ mask, const double* __restrict__ taupp,
                                            const StepArgs& a, const cx<T>* __restrict__ tw2,
                                            const cx<T>* __restrict__ tw3, double* __restrict__ partials,
@@ -602,7 +648,6 @@ __device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, double2* __res
   // threads past PPI*TPP (when TPP does not divide the block) sit out
   const int c_p0 = t < PPI2 * TPP2 ? t % PPI2 : (1 << 30), c_q = t / PPI2;
   const int r_p0 = t < PPI3 * TPP3 ? t / TPP3 : (1 << 30), r_q = t % TPP3;
-  double2* xb = xi + off;
   // All of this thread's xi values are loaded in one batch, so
   // their latency is paid once (one block per CU leaves no other block to
   // hide it).  Up to 4 points per thread the batch is issued before the
@@ -617,7 +662,7 @@ __device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, double2* __res
 #pragma unroll
     for (int k = 0; k < EPT; k++) {
       const int e = t + k * PLANE_THREADS;
-      if (BATCH && e < PLANE) xv[k] = xb[e];
+      if (BATCH && e < PLANE) xv[k] = xi.load(off + e);
     }
   };
   if (EARLY) fetch();
@@ -626,18 +671,18 @@ __device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, double2* __res
   // inverse along N3 (rows)
   fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)1, tw3);
   if (!EARLY) fetch();
-  const double nsize = (double)P;
+  const double inv_n = 1.0 / (double)P;
   const FieldCoef c = field_coef(a, f, step_ctr);
   double acc = 0.0;
   auto point = [&](int e, double2 x) {
     const int li = (e / N3) * LD + (e % N3);
     const cx<T> d = sm[li];
     if (!MASK || mask[off + e] == 0) {
-      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, (long)i1 * PLANE + e, nsize);
-      xb[e] = x;
+      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, (long)i1 * PLANE + e, inv_n);
+      xi.store(off + e, x);
     }
     sm[li] = {(T)x.x, (T)x.y};
-    acc += x.x / nsize;
+    acc += x.x;  // (the block sum is scaled by 1/P once)
   };
   if constexpr (BATCH) {
 #pragma unroll
@@ -646,18 +691,18 @@ __device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, double2* __res
       if (e < PLANE) point(e, xv[k]);
     }
   } else {
-    for (int e = t; e < PLANE; e += PLANE_THREADS) point(e, xb[e]);
+    for (int e = t; e < PLANE; e += PLANE_THREADS) point(e, xi.load(off + e));
   }
   __syncthreads();
   fft_pencils<T, N3, G3>(lrow, lrow, lrow, r_p0, PPI3, N2, r_q, (T)-1, tw3);
   fft_pencils<T, N2, G2>(lcol, lcol, gcol, c_p0, PPI2, N3, c_q, (T)-1, tw2);
-  const double sum = block_sum(acc, red);
+  const double sum = block_sum(acc, red) * inv_n;
   if (t == 0) partials[(long)f * n1 + i1] = sum;
 }
 
 template <typename T, int N2, int N3, bool MASK, bool TAUPP>
 __global__ void __launch_bounds__((plane_threads<T, N2, N3>()))
-k_plane_step(cx<T>* __restrict__ W, double2* __restrict__ xi, const uint8_t* __restrict__ mask,
+k_plane_step(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask,
              const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
              const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
   extern __shared__ unsigned char lds_raw[];
@@ -804,7 +849,7 @@ k_strain_spectrum(const cx<T>* __restrict__ W, cx<T>* __restrict__ E, const Elas
 // sigma = C:(eps - eps*) (stress G.c:897-919); 18 block sums -> partials.
 template <typename T, int SM>
 __global__ void __launch_bounds__(256)
-k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, const ElasticConsts* __restrict__ ec,
+k_strain_stress(const cx<T>* __restrict__ E, Master<T> xi, const ElasticConsts* __restrict__ ec,
                 const double* __restrict__ avepst, long P, double* __restrict__ partials, int nblk,
                 double* eps_field, double* sig_field) {
   __shared__ double red[4];
@@ -824,7 +869,7 @@ k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, con
 #pragma unroll
     for (int a = 0; a < SM; a++) {
       if (a >= S) continue;
-      const double x = xi[(long)a * P + pt].x;
+      const double x = xi.re((long)a * P + pt);
 #pragma unroll
       for (int kl = 0; kl < 9; kl++) es[kl] += ec->eps[a][kl] * x;
     }
@@ -858,7 +903,7 @@ k_strain_stress(const cx<T>* __restrict__ E, const double2* __restrict__ xi, con
 constexpr int ENS_THREADS = 1024, ENS_MAX_CLASSES = 8;
 struct EnsClassDev {
   void* W;                 // cx<T> [B*S][P]
-  double2* xi;
+  void* xi;                // the master (Master<T> over B*S*P points)
   const ElasticConsts* ec;
   const void* tw[3];       // cx<T> twiddles of the N1, N2, N3 axes
   StepArgs sa;
@@ -880,7 +925,8 @@ struct EnsDev {
 // those of the largest body rather than of all of them together.)
 template <typename T, int N>
 __device__ __noinline__ void ens_plane(const EnsClassDev* __restrict__ c, long local, cx<T>* sm) {
-  plane_body<T, N, N, false, false>(reinterpret_cast<cx<T>*>(c->W), c->xi, nullptr, nullptr, c->sa,
+  plane_body<T, N, N, false, false>(reinterpret_cast<cx<T>*>(c->W), make_master<T>(c->xi, (long)c->F * c->P),
+                                    nullptr, nullptr, c->sa,
                                     reinterpret_cast<const cx<T>*>(c->tw[1]),
                                     reinterpret_cast<const cx<T>*>(c->tw[2]), c->partials, c->step_ctr, local, sm);
 }
