@@ -9,15 +9,16 @@
  * (G.c:4450-4452, 1426-1447), and the GL loop (G.c:4565-4691) in fixed-step
  * mode or with plasticconverge (G.c:3732-3785).  interfacevolv and the
  * virtual-strain update are off (single material, xi_v = 0).  Log lines and
- * the files stress-strain.dat, phasefield.dat, pfout.dat, xi.dat and
- * xi_bc.dat keep the reference formats.
+ * the files stress-strain.dat, phasefield.dat, pfout.dat, strain1D.dat,
+ * stress1D.dat, strain.txt, stress.txt, xi.dat and xi_bc.dat keep the
+ * reference formats.
  *
  *   pfdd [preset] [key=value ...]
  * presets: c0 (32^3 x 3, one loop, 500 steps), c1 (128^3 x 12, 16 loops,
  * 1000 steps), c2 (256^3 x 12, 64 loops, 2000 steps, mixed precision).
  * keys: n1 n2 n3 systems(plane111|fcc) nloops rmin rmax seed sigma_amp tau0
  *       steps nt converge plastic_max precision(fp64|mixed) bmode(recompute|stored)
- *       quirk pfout_every outdir
+ *       quirk pfout_every fields(1|0) outdir
  */
 #include <math.h>
 #include <stdio.h>
@@ -50,6 +51,7 @@ typedef struct {
   int plastic_max;
   int precision, bmode, quirk;
   int pfout_every;      /* 0: no pfout.dat */
+  int fields;           /* 1: strain1D.dat / stress1D.dat per strain call, strain.txt / stress.txt at the end */
   char outdir[512];
 } config;
 
@@ -59,6 +61,7 @@ static void preset(config *c, const char *name)
   c->nt = 3;
   c->plastic_max = 3000;
   c->quirk = 1;
+  c->fields = 1;
   c->precision = PFDD_FP64;
   c->bmode = PFDD_B_RECOMPUTE;
   c->seed = 20261202ULL;
@@ -116,11 +119,65 @@ static void set_key(config *c, const char *kv)
   else if (!strcmp(key, "bmode")) c->bmode = !strcmp(v, "stored") ? PFDD_B_STORED : PFDD_B_RECOMPUTE;
   else if (!strcmp(key, "quirk")) c->quirk = atoi(v);
   else if (!strcmp(key, "pfout_every")) c->pfout_every = atoi(v);
+  else if (!strcmp(key, "fields")) c->fields = atoi(v);
   else if (!strcmp(key, "outdir")) snprintf(c->outdir, sizeof(c->outdir), "%s", v);
   else {
     fprintf(stderr, "pfdd: unknown key %s\n", key);
     exit(2);
   }
+}
+
+/* Field outputs of calc_strain_stress (G.c:4039-4085).  e and s are the
+ * strain and stress fields [3][3][N1][N2][N3] of pfdd_strain_stress; mu
+ * divides the stress for the sigma columns. */
+static double fld(const double *f, int i, int j, long pt, long P) { return f[(long)(3 * i + j) * P + pt]; }
+
+/* strain1D.dat / stress1D.dat: the 33 component along N1 at (N2/2, N3/2)
+ * (G.c:4074-4083; the zone header names N3 as the reference's does). */
+static void write_1d(FILE *f, const double *x, double scale, int n1, int n2, int n3)
+{
+  const long P = (long)n1 * n2 * n3;
+  fprintf(f, "zone   I = %d \n", n3);
+  for (int k1 = 0; k1 < n1; k1++)
+    fprintf(f, "%d %lf \n", k1, fld(x, 2, 2, ((long)k1 * n2 + n2 / 2) * n3 + n3 / 2, P) * scale);
+}
+
+/* strain.txt (G.c:818-831): every point, the nine components; the header is
+ * the reference's (eight names for nine columns). */
+static void write_strain_txt(FILE *f, const double *e, int n1, int n2, int n3)
+{
+  const long P = (long)n1 * n2 * n3;
+  fprintf(f, "x,y,z,eps11,eps13,eps21,eps22,eps23,eps31,eps32,eps33\n");
+  for (int k1 = 0; k1 < n1; k1++)
+    for (int k2 = 0; k2 < n2; k2++)
+      for (int k3 = 0; k3 < n3; k3++) {
+        const long pt = ((long)k1 * n2 + k2) * n3 + k3;
+        fprintf(f, "%d,%d,%d,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf \n", k1, k2, k3, fld(e, 0, 0, pt, P),
+                fld(e, 0, 1, pt, P), fld(e, 0, 2, pt, P), fld(e, 1, 0, pt, P), fld(e, 1, 1, pt, P),
+                fld(e, 1, 2, pt, P), fld(e, 2, 0, pt, P), fld(e, 2, 1, pt, P), fld(e, 2, 2, pt, P));
+      }
+}
+
+/* stress.txt (G.c:927-944): the k2 = 0 plane.  The reference's first six
+ * columns are the virtual-field stress sigmav = sigma_applied - IFFT(sum_b
+ * B_bv xi_b)/P of virtualevolv (G.c:1455-1464); with the virtual eigenstrain
+ * e_u (x) e_v, sum_b B_bv xi_b(k) = -sigma_uv(k)/mu for k != 0, so that
+ * field is the stress C:(eps - eps*) over mu, which is what is written; the
+ * last six columns are C:(eps - eps*) itself (datasigma). */
+static void write_stress_txt(FILE *f, const double *s, double mu, int n1, int n2, int n3)
+{
+  const long P = (long)n1 * n2 * n3;
+  const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+  fprintf(f, "x,y,z,sigma11,sigma12,sigma13,sigma22,sigma23,sigma33,datasigma11,datasigma12,datasigma13,"
+             "datasigma22,datasigma23,datasigma33 \n");
+  for (int k1 = 0; k1 < n1; k1++)
+    for (int k3 = 0; k3 < n3; k3++) {
+      const long pt = (long)k1 * n2 * n3 + k3;  /* k2 = 0 */
+      fprintf(f, "%d,%d,%d", k1, 0, k3);
+      for (int t = 0; t < 6; t++) fprintf(f, ",%lf", fld(s, ii[t], jj[t], pt, P) / mu);
+      for (int t = 0; t < 6; t++) fprintf(f, ",%lf", fld(s, ii[t], jj[t], pt, P));
+      fprintf(f, "\n");
+    }
 }
 
 static FILE *open_out(const config *c, const char *name, const char *mode)
@@ -297,6 +354,14 @@ int main(int argc, char **argv)
   FILE *of2 = open_out(&cfg, "stress-strain.dat", "w");
   FILE *ofpfv = open_out(&cfg, "phasefield.dat", "w");
   FILE *ofpfout = cfg.pfout_every > 0 ? open_out(&cfg, "pfout.dat", "w") : NULL;
+  FILE *of4 = cfg.fields ? open_out(&cfg, "strain1D.dat", "w") : NULL;
+  FILE *of6 = cfg.fields ? open_out(&cfg, "stress1D.dat", "w") : NULL;
+  double *efield = cfg.fields ? malloc(sizeof(double) * 9 * (size_t)P) : NULL;
+  double *sfield = cfg.fields ? malloc(sizeof(double) * 9 * (size_t)P) : NULL;
+  if (cfg.fields && (!efield || !sfield)) {
+    fprintf(stderr, "pfdd: out of host memory for the strain/stress fields\n");
+    return 1;
+  }
   double ap_u[9];
   avepst_uniform(&p, sigma, ap_u);
   int maxsteps = cfg.converge ? cfg.plastic_max : cfg.steps;
@@ -315,9 +380,23 @@ int main(int argc, char **argv)
     /* calc_strain_stress (G.c:4044) */
     if (it == cfg.nt - 1 || (it != 0 && it != cfg.nt - 1)) {
       double aveps[9], avesig[9];
-      die(pfdd_strain_stress(ctx, ap_u, aveps, avesig, NULL, NULL), "pfdd_strain_stress");
+      die(pfdd_strain_stress(ctx, ap_u, aveps, avesig, efield, sfield), "pfdd_strain_stress");
+      if (cfg.fields && it == cfg.nt - 1) {
+        printf("printing strains %d %d\n", it, itp);
+        FILE *of3 = open_out(&cfg, "strain.txt", "w");
+        write_strain_txt(of3, efield, p.n1, p.n2, p.n3);
+        fclose(of3);
+        printf("printing stress %d %d\n", it, itp);
+        FILE *of5 = open_out(&cfg, "stress.txt", "w");
+        write_stress_txt(of5, sfield, mu, p.n1, p.n2, p.n3);
+        fclose(of5);
+      }
       for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) fprintf(of2, "%e %e ", aveps[3 * i + j], avesig[3 * i + j] / mu);
+      if (cfg.fields) {
+        write_1d(of4, efield, 1.0, p.n1, p.n2, p.n3);
+        write_1d(of6, sfield, 1.0 / mu, p.n1, p.n2, p.n3);
+      }
     }
     /* fft_forward + calc_data2_datag + virtualevolv data2 term */
     die(pfdd_begin_outer(ctx, cfg.quirk), "pfdd_begin_outer");
@@ -377,6 +456,10 @@ int main(int argc, char **argv)
   fclose(of2);
   fclose(ofpfv);
   if (ofpfout) fclose(ofpfout);
+  if (of4) fclose(of4);
+  if (of6) fclose(of6);
+  free(efield);
+  free(sfield);
   pfdd_destroy(ctx);
   if (gl_time > 0)
     printf("pfdd: %ld GL steps in %.3f s (%.1f GL steps/s)\n", total_steps, gl_time, total_steps / gl_time);

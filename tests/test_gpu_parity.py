@@ -401,6 +401,30 @@ def test_driver_c0_matches_oracle(tmp_path):
     assert len(ss) == 2 * 18 + w.ns + 1                               # two strain/stress calls + gammas
     bc = np.fromfile(tmp_path / "xi_bc.dat", dtype=np.float64).reshape(w.ns + 9, *w.shape, 2)
     assert np.all(bc[:w.ns] == 0) and np.all(bc[w.ns:, ..., 0] == 1)
+    # field outputs of the last strain/stress call (it = NT-1, after the GL loop) against the oracle's fields
+    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * w.ns, C.asf2_reference(w.ns), [1.0] * w.ns)
+    o = O.Oracle(p)
+    o.set_xi(w.xi)
+    o.set_tau(w.tau)
+    o.prologue(True)
+    o.gl_steps(40)
+    _, _, ef, sf = o.strain_stress(w.s_tensor_avepst(), fields=True)
+    n1, n2, n3 = w.shape
+    st = np.loadtxt(tmp_path / "strain.txt", delimiter=",", skiprows=1)
+    assert st.shape == (n1 * n2 * n3, 12)
+    assert np.array_equal(st[:, 0], np.repeat(np.arange(n1), n2 * n3))
+    assert np.abs(st[:, 3:] - ef.reshape(9, -1).T).max() <= 6e-7          # printed with %lf
+    ss = np.loadtxt(tmp_path / "stress.txt", delimiter=",", skiprows=1)
+    assert ss.shape == (n1 * n3, 15) and np.all(ss[:, 1] == 0)
+    sym = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    want = np.stack([sf[i, j, :, 0, :].reshape(-1) for i, j in sym], axis=1)
+    assert np.abs(ss[:, 3:9] - want / C.C44).max() <= 6e-7 and np.abs(ss[:, 9:] - want).max() <= 6e-7
+    for name, f, scale in (("strain1D.dat", ef, 1.0), ("stress1D.dat", sf, 1.0 / C.C44)):
+        zones = open(tmp_path / name).read().split("zone")
+        assert len(zones) == 3                                            # two strain/stress calls
+        last = np.array([[float(v) for v in l.split()] for l in zones[-1].splitlines()[1:] if l.strip()])
+        assert np.array_equal(last[:, 0], np.arange(n1))
+        assert np.abs(last[:, 1] - scale * f[2, 2, :, n2 // 2, n3 // 2]).max() <= 6e-7
 
 
 def test_driver_converge_mode_runs():
