@@ -23,6 +23,10 @@
  *   orc_strain_stress    compiled reference strain()/stress() at 32x2x32, and
  *                        <sigma> = sigma_applied*mu for periodic cells
  *                        [test_oracle_strain.py]
+ *   orc_line_points,     compiled reference avestrain (energy_stressTimePf,
+ *   orc_line_energy,     energy_in4) and ResidualEnergy at 32x2x32 on random
+ *   orc_residual_energy  xi, stresses and D coefficients; a brute-force
+ *                        restatement of the line rule [test_oracle_energy.py]
  */
 #include "pfdd_oracle.h"
 
@@ -659,3 +663,73 @@ void orc_strain_stress(orc_state *s, const double *avepst_uniform, double *aveps
   }
   free(beta);
 }
+
+/* ======================================================================
+ * Energy diagnostics on the interface line (avestrain, ResidualEnergy).
+ * ====================================================================== */
+
+int orc_line_points(int n1, int n2, int n3, const double n[3], const int pp[3], long *pts, int cap)
+{
+  int cnt = 0;
+  const int k1 = pp[0];
+  if (k1 < 0 || k1 >= n1) return 0;
+  for (int k2 = 0; k2 < n2; k2++)
+    for (int k3 = 0; k3 < n3; k3++)
+      if (fabs(n[0] * (k1 - pp[0]) + n[1] * (k2 - pp[1]) + n[2] * (k3 - pp[2])) <= 1.0) {
+        if (cnt < cap) pts[cnt] = ((long)k1 * n2 + k2) * n3 + k3;
+        cnt++;
+      }
+  return cnt;
+}
+
+/* S_ijkl of avestrain (G.c:656-666) */
+static double s_ijkl(int i, int j, int k, int l, double S11, double S12, double S44)
+{
+  const double dij = i == j, dkl = k == l, dik = i == k, djl = j == l, dil = i == l, djk = j == k;
+  const double d4 = (i == j && j == k && k == l) ? 1.0 : 0.0;
+  return (S11 + 2 * S12) / 3.0 * dij * dkl + S44 / 4.0 * (dik * djl + dil * djk) - S44 / 2.0 * d4 +
+         (S11 - S12) * (d4 - dij * dkl / 3.0);
+}
+
+double orc_line_energy(const orc_state *s, const double sigma[9], double S11, double S12, double S44,
+                       double mu, const long *pts, int npts, double *e4_inc)
+{
+  const int S = s->p.ns;
+  double e3 = 0.0, e4 = 0.0;
+  double ssig = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) ssig += s_ijkl(i, j, k, l, S11, S12, S44) * sigma[3 * k + l] * sigma[3 * i + j];
+  for (int q = 0; q < npts; q++) {
+    const long pt = pts[q];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        double sp = 0.0;
+        for (int a = 0; a < S; a++) sp += s->xi[2 * ((long)a * s->P + pt)] * s->p.eps[a][3 * i + j];
+        e3 += -sp * sigma[3 * i + j];
+      }
+    e4 += -ssig * 0.5 * mu;
+  }
+  if (e4_inc) *e4_inc = e4;
+  return e3;
+}
+
+double orc_residual_energy(const orc_state *s, const int pp[3], int ns1, double D00, double D01, double D10,
+                           double D11)
+{
+  const int S = s->p.ns, n2 = s->p.n2, n3 = s->p.n3;
+  double en = 0.0;
+  for (int j = 0; j < n2; j++) {
+    const long pt = ((long)pp[0] * n2 + j) * n3 + pp[2];
+    double x0 = 0.0, x1 = 0.0;
+    for (int a = 0; a < S; a++) {
+      const int m = a / ns1;
+      if (m == 0) x0 += s->xi[2 * ((long)a * s->P + pt)];
+      if (m == 1) x1 += s->xi[2 * ((long)a * s->P + pt)];
+    }
+    en += fabs(-D11 * x1 * x1 + (D10 + D01) * x1 * x0 - D00 * x0 * x0);
+  }
+  return en;
+}
+

@@ -107,6 +107,26 @@ double orc_plasticevolv(int ns, long P, double *xi, const double *xi_bc, const d
 void orc_strain_stress(orc_state *s, const double *avepst_uniform,
                        double *aveps, double *avesigma, double *eps_field, double *sig_field);
 
+/* ---- energy diagnostics on the interface line ---------------------------
+ * Points with k1 == pp[0] and |n.(k - pp)| <= 1 (avestrain G.c:707/714,
+ * Energy_calculation G.c:3440), in k1, k2, k3 order as point indices
+ * (k1 N2 + k2) N3 + k3; at most cap are written; returns the count. */
+int orc_line_points(int n1, int n2, int n3, const double n[3], const int pp[3], long *pts, int cap);
+
+/* avestrain's energies (G.c:641-749) for a uniform applied stress sigma[9]
+ * (units of mu) and zero virtual strains: returns energy_stressTimePf =
+ * -sum_{line} sum_ij strain_p,ij sigma_ij, strain_p = sum_a Re xi_a e_a;
+ * *e4_inc = -sum_{line} sum_ijkl S_ijkl sigma_kl sigma_ij mu / 2, what each
+ * call adds to energy_in4. */
+double orc_line_energy(const orc_state *s, const double sigma[9], double S11, double S12, double S44,
+                       double mu, const long *pts, int npts, double *e4_inc);
+
+/* ResidualEnergy (G.c:3920-3947): sum over k2 of
+ * |-D11 x1^2 + (D10 + D01) x1 x0 - D00 x0^2| at (pp[0], k2, pp[2]), x_m the
+ * sum of Re xi_a over the systems of material m (a / ns1 == m). */
+double orc_residual_energy(const orc_state *s, const int pp[3], int ns1, double D00, double D01, double D10,
+                           double D11);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,12 @@ def lib():
         L.orc_plasticevolv.argtypes = [ci, ctypes.c_long, vp, vp, vp, vp, vp, vp, vp, ci, cd, cd, vp]
         L.orc_plasticevolv.restype = cd
         L.orc_strain_stress.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_line_points.argtypes = [ci, ci, ci, vp, vp, vp, ci]
+        L.orc_line_points.restype = ci
+        L.orc_line_energy.argtypes = [vp, vp, cd, cd, cd, cd, vp, ci, vp]
+        L.orc_line_energy.restype = cd
+        L.orc_residual_energy.argtypes = [vp, vp, ci, cd, cd, cd, cd]
+        L.orc_residual_energy.restype = cd
         _lib = L
     return _lib
 
@@ -186,6 +192,19 @@ class Oracle:
         lib().orc_gl_steps(self.h, n, g.ctypes.data, ga.ctypes.data)
         return g, ga
 
+    def line_energy(self, sigma, S11, S12, S44, mu, pts):
+        """(energy_stressTimePf, energy_in4 increment) of avestrain on the line points."""
+        sg = np.ascontiguousarray(sigma, dtype=np.float64).reshape(9)
+        p = np.ascontiguousarray(pts, dtype=np.int64)
+        e4 = ctypes.c_double()
+        e3 = lib().orc_line_energy(self.h, sg.ctypes.data, S11, S12, S44, mu, p.ctypes.data, len(p),
+                                   ctypes.byref(e4))
+        return e3, e4.value
+
+    def residual_energy(self, ppoint, ns1, D00, D01, D10, D11):
+        pp = np.ascontiguousarray(ppoint, dtype=np.int32)
+        return lib().orc_residual_energy(self.h, pp.ctypes.data, ns1, D00, D01, D10, D11)
+
     def strain_stress(self, avepst_uniform, fields=False):
         ap = np.ascontiguousarray(avepst_uniform, dtype=np.float64).reshape(9)
         aveps = np.zeros(9)
@@ -198,6 +217,16 @@ class Oracle:
         if fields:
             return aveps.reshape(3, 3), avesig.reshape(3, 3), ef.reshape((3, 3) + self.shape), sf.reshape((3, 3) + self.shape)
         return aveps.reshape(3, 3), avesig.reshape(3, 3)
+
+
+def line_points(shape, normal, ppoint):
+    """Point indices of the interface line of the energy diagnostics."""
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    pp = np.ascontiguousarray(ppoint, dtype=np.int32)
+    cnt = lib().orc_line_points(*shape, n.ctypes.data, pp.ctypes.data, None, 0)
+    pts = np.zeros(max(cnt, 1), dtype=np.int64)
+    lib().orc_line_points(*shape, n.ctypes.data, pp.ctypes.data, pts.ctypes.data, cnt)
+    return pts[:cnt]
 
 
 def plasticevolv(xi, d2, cd2, asf2, dslip, tau, xi_bc=None, tau_per_point=False, uq2=1.0, pi_const=3.141592654):

@@ -48,6 +48,10 @@ def lib():
         L.setorient.argtypes = [vp]
         L.set3D1sys.argtypes = [vp, vp]
         L.resolSS.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.avestrain.argtypes = [vp, vp, vp, vp, vp, ci, vp, cd, cd, cd, cd, vp, vp, ci, vp, vp]
+        L.avestrain.restype = cf
+        L.ResidualEnergy.argtypes = [vp, vp, ci, ci, ci, cd, cd, cd, cd]
+        L.ResidualEnergy.restype = cf
         _lib = L
     return _lib
 
@@ -259,3 +263,33 @@ def gl_loop(xi0, eps, epsv, C11, C12, C44, cd2, asf2, dslip, tau, nsteps):
         for a in range(NS):
             d2[a] = fourn3d(d2[a].reshape(N1, N2, N3), 1).reshape(P)
     return xi[:NS], np.array(gammas)
+
+
+def avestrain(xi, eps, sigma, S11, S12, S44, mu, border, normal, ppoint):
+    """The reference avestrain on xi [NSV][N1][N2][N3] (complex) with a uniform
+    applied stress sigma[3][3]: (energy_stressTimePf, energy_in4 added)."""
+    x = np.zeros((NSV, N1, N2, N3, 2))
+    x[..., 0] = np.asarray(xi).real
+    x[..., 1] = np.asarray(xi).imag
+    avepsd = np.zeros((3, 3))
+    avepst = np.zeros((P, 3, 3))
+    e = np.ascontiguousarray(eps, dtype=np.float64).reshape(NS, 3, 3)
+    ev = epsv_basis()
+    sg = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), (P, 3, 3)))
+    e4 = ctypes.c_float(0.0)
+    sa = ctypes.c_float(0.0)
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    pp = np.ascontiguousarray(ppoint, dtype=np.int32)
+    r = lib().avestrain(avepsd.ctypes.data, avepst.ctypes.data, e.ctypes.data, ev.ctypes.data, x.ctypes.data, P,
+                        sg.ctypes.data, S11, S12, S44, mu, ctypes.byref(e4), ctypes.byref(sa), border,
+                        n.ctypes.data, pp.ctypes.data)
+    return float(r), float(e4.value)
+
+
+def residual_energy(xi, normal, ppoint, D00, D01, D10, D11):
+    x = np.zeros((NSV, N1, N2, N3, 2))
+    x[..., 0] = np.asarray(xi).real
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    return float(lib().ResidualEnergy(x.ctypes.data, n.ctypes.data, int(ppoint[0]), int(ppoint[1]),
+                                      int(ppoint[2]), D00, D01, D10, D11))
+
