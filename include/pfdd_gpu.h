@@ -127,6 +127,24 @@ int pfdd_begin_outer(pfdd_ctx *c, int quirk);
  * gamma_a is non-NULL, asynchronous otherwise. */
 int pfdd_gl_steps(pfdd_ctx *c, int nsteps, double *gamma, double *gamma_a);
 
+/* Interface-line diagnostics.  avestrain (G.c:697-718) sums its
+ * stress x plastic-strain energy (energy_in3) and applied-stress energy
+ * (energy_in4) over the grid points with k1 == ppoint[0] and
+ * |interface_n . (k - ppoint)| <= 1; with a uniform applied stress and no
+ * virtual strain, energy_in3 = -sum_a (e_a : sigma) sum_line Re xi_a, so the
+ * device only needs the line sums of Re xi_a.
+ *
+ * pfdd_set_line_points sets the point list of the context: n flat indices
+ * (k1 N2 + k2) N3 + k3 in [0, N1 N2 N3) (pfdd_line_points builds the
+ * reference's list); n = 0 clears it.  While a list is set every GL step also
+ * writes the line sums of that step.  pfdd_line_sums returns sums[ns] for the
+ * current xi (synchronous); pfdd_gl_steps_lines is pfdd_gl_steps that also
+ * returns line_sums[nsteps][ns], the sums after each step (synchronous,
+ * PFDD_E_STATE without a list). */
+int pfdd_set_line_points(pfdd_ctx *c, const long long *pts, int n);
+int pfdd_line_sums(pfdd_ctx *c, double *sums);
+int pfdd_gl_steps_lines(pfdd_ctx *c, int nsteps, double *gamma, double *gamma_a, double *line_sums);
+
 /* Strain and stress of the current xi (strain G.c:752-835, stress
  * G.c:837-972, averages G.c:4058-4070).  avepst_uniform[3][3] is the part
  * of avepst that does not depend on xi, S:sigma_applied*mu (avestrain
@@ -285,6 +303,12 @@ long pfdd_add_loop(double *xi, int n1, int n2, int n3, int sys, const double nor
  * [nloops][6]) receives a, c, r and the number of points set. */
 int pfdd_seed_loops(double *xi, int n1, int n2, int n3, int ns, const double (*normals)[3], int nloops,
                     double rmin, double rmax, unsigned long long *rng_state, double *loops_out);
+
+/* The interface line of avestrain (G.c:707, 714): the points with
+ * k1 == ppoint[0] and |normal . (k - ppoint)| <= 1, in k2, k3 order, as flat
+ * indices (k1 N2 + k2) N3 + k3.  Writes at most cap of them to pts (may be
+ * NULL) and returns the count. */
+int pfdd_line_points(int n1, int n2, int n3, const double normal[3], const int ppoint[3], long long *pts, int cap);
 
 /* Symmetric applied stress with the six independent components (11, 12, 13,
  * 22, 23, 33) drawn in that order from U[-amp, amp). */

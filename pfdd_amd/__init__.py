@@ -56,7 +56,7 @@ EXPORTED = [
     "pfdd_ensemble_set_tau", "pfdd_ensemble_upload_xi", "pfdd_ensemble_download_xi", "pfdd_ensemble_begin_outer",
     "pfdd_ensemble_gl_steps", "pfdd_ensemble_autotune", "pfdd_ensemble_set_plan", "pfdd_ensemble_sync",
     "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes", "pfdd_time_step_kernels", "pfdd_ensemble_set_mode",
-    "pfdd_ensemble_get_mode",
+    "pfdd_ensemble_get_mode", "pfdd_set_line_points", "pfdd_line_sums", "pfdd_gl_steps_lines", "pfdd_line_points",
 ]
 ENS_PHASED, ENS_PERSISTENT = 0, 1
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
@@ -142,6 +142,10 @@ def load():
     L.pfdd_fft_host_release.restype = None
     fp = ctypes.POINTER(ctypes.c_float)
     L.pfdd_time_step_kernels.argtypes = [vp, ci, vp, vp]
+    L.pfdd_set_line_points.argtypes = [vp, vp, ci]
+    L.pfdd_line_sums.argtypes = [vp, vp]
+    L.pfdd_gl_steps_lines.argtypes = [vp, ci, vp, vp, vp]
+    L.pfdd_line_points.argtypes = [ci, ci, ci, vp, vp, vp, ci]
     L.pfdd_set_plan.argtypes = [vp, ci]
     L.pfdd_get_plan.argtypes = [vp]
     L.pfdd_autotune.argtypes = [vp, ci, vp, fp]
@@ -258,6 +262,27 @@ class Context:
         ga = np.zeros((n, self.ns))
         _check(load().pfdd_gl_steps(self.h, n, g.ctypes.data, ga.ctypes.data), "pfdd_gl_steps")
         return g, ga
+
+    def set_line_points(self, pts):
+        """Point list (flat indices) whose Re xi sums every GL step reports; [] clears it."""
+        q = np.ascontiguousarray(pts, dtype=np.int64).reshape(-1)
+        _check(load().pfdd_set_line_points(self.h, q.ctypes.data if q.size else None, int(q.size)),
+               "pfdd_set_line_points")
+
+    def line_sums(self):
+        """Sums of Re xi_a over the point list for the current xi: [ns]."""
+        out = np.zeros(self.ns)
+        _check(load().pfdd_line_sums(self.h, out.ctypes.data), "pfdd_line_sums")
+        return out
+
+    def gl_steps_lines(self, n):
+        """gl_steps that also returns the line sums after every step: (gamma, gamma_a, lines [n][ns])."""
+        g = np.zeros(n)
+        ga = np.zeros((n, self.ns))
+        ln = np.zeros((n, self.ns))
+        _check(load().pfdd_gl_steps_lines(self.h, n, g.ctypes.data, ga.ctypes.data, ln.ctypes.data),
+               "pfdd_gl_steps_lines")
+        return g, ga, ln
 
     def strain_stress(self, avepst_uniform, fields=False):
         ap = np.ascontiguousarray(avepst_uniform, dtype=np.float64).reshape(9)

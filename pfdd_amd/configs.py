@@ -84,6 +84,26 @@ def add_loop(xi, sys, normal, center, radius):
     return load().pfdd_add_loop(xi.ctypes.data, n1, n2, n3, sys, nrm.ctypes.data, c.ctypes.data, float(radius))
 
 
+# Interface normal and penetration point of main (G.c:4200-4203, 4234-4236).
+INTERFACE_N = (np.sqrt(3) / 3.0, 0.0, np.sqrt(6) / 3.0)
+
+
+def interface_ppoint(shape):
+    n1, _, n3 = shape
+    return (n1 // 2, 0, n3 * 3 // 4)
+
+
+def line_points(shape, normal=INTERFACE_N, ppoint=None):
+    """pfdd_line_points: flat indices of the interface line of avestrain."""
+    n1, n2, n3 = shape
+    pp = np.ascontiguousarray(interface_ppoint(shape) if ppoint is None else ppoint, dtype=np.int32)
+    nrm = np.ascontiguousarray(normal, dtype=np.float64)
+    cnt = load().pfdd_line_points(n1, n2, n3, nrm.ctypes.data, pp.ctypes.data, None, 0)
+    out = np.zeros(max(cnt, 1), dtype=np.int64)
+    load().pfdd_line_points(n1, n2, n3, nrm.ctypes.data, pp.ctypes.data, out.ctypes.data, cnt)
+    return out[:cnt]
+
+
 # Orientation whose rows are [-110]/sqrt2, [-1-12]/sqrt6, [111]/sqrt3: the
 # (111) normal maps onto z, so (111) loops lie in grid planes k = const.
 ROWS_111_TO_Z = [[-1 / np.sqrt(2), 1 / np.sqrt(2), 0.0],

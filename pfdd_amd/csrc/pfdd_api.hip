@@ -72,6 +72,9 @@ struct pfdd_ctx {
   void* E6 = nullptr;        // strain spectrum [6][P], allocated on first use
   double* efield = nullptr;  // [9][P] strain/stress fields, on demand
   double* sfield = nullptr;
+  long* line_pts = nullptr;  // interface-line point list (pfdd_set_line_points)
+  int nline = 0;
+  double* line_out = nullptr; // [HIST_CAP][F] line sums per step; row 0 also serves pfdd_line_sums
   Twiddles tw[3];
   StepArgs sa{};
   int upd_nblk = 0;
@@ -473,7 +476,20 @@ int prologue(pfdd_ctx* c, int quirk) {
   return PFDD_OK;
 }
 
-int step(pfdd_ctx* c) { return c->fp64 ? one_step<double>(c) : one_step<float>(c); }
+template <typename T>
+void line_sums(pfdd_ctx* c, const int* step_ctr) {
+  hipLaunchKernelGGL((k_line_sums<T>), dim3((unsigned)c->F), dim3(UPD_THREADS), 0, c->stream, xm<T>(c), c->P,
+                     (const long*)c->line_pts, c->nline, c->F, c->line_out, step_ctr);
+}
+
+int step(pfdd_ctx* c) {
+  const int rc = c->fp64 ? one_step<double>(c) : one_step<float>(c);
+  if (rc || c->nline == 0) return rc;
+  if (c->fp64) line_sums<double>(c, c->step_ctr);
+  else line_sums<float>(c, c->step_ctr);
+  tap(c, PFDD_STAGE_FINISH);
+  return PFDD_OK;
+}
 
 int make_twiddles(Twiddles& t, int n, long long& bytes, hipStream_t st) {
   static const long double TWO_PI = 6.283185307179586476925286766559005768L;
@@ -614,7 +630,8 @@ void pfdd_destroy(pfdd_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   void* bufs[] = {c->xi,   c->W,        c->Bsym, c->mask, c->taupp, c->taud,   c->partials, c->hist,
-                  c->step_ctr, c->small, c->avepst_d, c->ec, c->E6,  c->efield, c->sfield,   c->freq};
+                  c->step_ctr, c->small, c->avepst_d, c->ec, c->E6,  c->efield, c->sfield,   c->freq,
+                  c->line_pts, c->line_out};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& t : c->tw) {
@@ -1301,6 +1318,70 @@ int pfdd_gl_steps(pfdd_ctx* c, int nsteps, double* gamma, double* gamma_a) {
   return gl_steps_ctx(c, nsteps, gamma, gamma_a);
 }
 
+int pfdd_set_line_points(pfdd_ctx* c, const long long* pts, int n) {
+  if (!c || n < 0 || (long)n > c->P || (n > 0 && !pts)) return PFDD_E_SHAPE;
+  for (int i = 0; i < n; i++)
+    if (pts[i] < 0 || pts[i] >= c->P) return PFDD_E_SHAPE;
+  HIPCK(hipStreamSynchronize(c->stream));
+  drop_graph(c);  // the captured steps hold the old list (or none)
+  if (c->line_pts) {
+    (void)hipFree(c->line_pts);
+    (void)hipFree(c->line_out);
+    c->bytes -= (long long)(sizeof(long) * c->nline + sizeof(double) * HIST_CAP * c->F);
+    c->line_pts = nullptr;
+    c->line_out = nullptr;
+    c->nline = 0;
+  }
+  if (n == 0) return PFDD_OK;
+  static_assert(sizeof(long) == sizeof(long long), "64-bit point indices");
+  c->line_pts = (long*)dev_alloc<char>(sizeof(long) * n, c->bytes);
+  c->line_out = (double*)dev_alloc<char>(sizeof(double) * HIST_CAP * c->F, c->bytes);
+  if (!c->line_pts || !c->line_out) {
+    (void)hipFree(c->line_pts);
+    (void)hipFree(c->line_out);
+    c->line_pts = nullptr;
+    c->line_out = nullptr;
+    return PFDD_E_ALLOC;
+  }
+  HIPCK(hipMemcpy(c->line_pts, pts, sizeof(long) * n, hipMemcpyHostToDevice));
+  c->nline = n;
+  return PFDD_OK;
+}
+
+int pfdd_line_sums(pfdd_ctx* c, double* sums) {
+  if (!c || !sums) return PFDD_E_SHAPE;
+  if (!c->has_xi || c->nline == 0) return PFDD_E_STATE;
+  if (c->fp64) line_sums<double>(c, nullptr);
+  else line_sums<float>(c, nullptr);
+  HIPCK(hipPeekAtLastError());
+  HIPCK(hipMemcpyAsync(sums, c->line_out, sizeof(double) * c->F, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return PFDD_OK;
+}
+
+int pfdd_gl_steps_lines(pfdd_ctx* c, int nsteps, double* gamma, double* gamma_a, double* line_sums) {
+  if (!c || nsteps < 0 || !line_sums) return PFDD_E_SHAPE;
+  if (!c->prologue_done || c->nline == 0) return PFDD_E_STATE;
+  const int S = c->S, F = c->F;
+  int copy_rc = PFDD_OK;
+  std::vector<double> rows;
+  const int rc = run_steps(&c, 1, c->stream, &c->graph, nsteps, true, [&](int, int first, int n, const double* h) {
+    for (int s = 0; s < n; s++) {
+      if (gamma) gamma[first + s] = h[(size_t)s * (S + 1) + S];
+      if (gamma_a)
+        for (int a = 0; a < S; a++) gamma_a[(size_t)(first + s) * S + a] = h[(size_t)s * (S + 1) + a];
+    }
+    rows.resize((size_t)n * F);
+    if (copy_rc == PFDD_OK &&
+        (hipMemcpyAsync(rows.data(), c->line_out, sizeof(double) * rows.size(), hipMemcpyDeviceToHost, c->stream) !=
+             hipSuccess ||
+         hipStreamSynchronize(c->stream) != hipSuccess))
+      copy_rc = PFDD_E_HIP;
+    if (copy_rc == PFDD_OK) std::copy(rows.begin(), rows.end(), line_sums + (size_t)first * F);
+  });
+  return rc ? rc : copy_rc;
+}
+
 int pfdd_strain_stress(pfdd_ctx* c, const double* avepst_uniform, double* aveps, double* avesigma,
                        double* eps_field, double* sig_field) {
   if (!c || !avepst_uniform || !aveps || !avesigma) return PFDD_E_SHAPE;
@@ -1391,7 +1472,7 @@ extern "C" int pfdd_step_plan(const pfdd_ctx* c, int* launches) {
   launches[PFDD_STAGE_CONTRACT] = 1;
   launches[PFDD_STAGE_STEP] = 0;
   launches[PFDD_STAGE_COPY] = 0;
-  launches[PFDD_STAGE_FINISH] = 1;
+  launches[PFDD_STAGE_FINISH] = c->nline > 0 ? 2 : 1;  // + the line sums while a list is set
   return PFDD_OK;
 }
 

@@ -391,6 +391,22 @@ k_xi_sums(Master<T> xi, long P, int nblk, double* __restrict__ partials) {
   if (threadIdx.x == 0) partials[(long)f * nblk + blockIdx.x] = s;
 }
 
+// Sum of Re xi over a point list per field (the interface line of avestrain,
+// G.c:697-718).  grid = F blocks.  The result goes to row step_ctr[0] - 1 of
+// out[rows][F] (the step that has just finished: the step's last reduction
+// advances step_ctr[0]) or, without step_ctr, to row 0.
+template <typename T>
+__global__ void __launch_bounds__(UPD_THREADS)
+k_line_sums(Master<T> xi, long P, const long* __restrict__ pts, int npts, int F, double* __restrict__ out,
+            const int* __restrict__ step_ctr) {
+  __shared__ double red[UPD_THREADS / 64];
+  const int f = blockIdx.x;
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < npts; q += UPD_THREADS) acc += xi.re((long)f * P + pts[q]);
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[(long)(step_ctr ? step_ctr[0] - 1 : 0) * F + f] = s;
+}
+
 template <typename T>
 __global__ void k_xi_to_w(Master<T> xi, cx<T>* __restrict__ W, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -142,3 +142,22 @@ void pfdd_seed_sigma(double amp, unsigned long long *rng_state, double sigma[3][
     sigma[ij[t][1]][ij[t][0]] = v;
   }
 }
+
+int pfdd_line_points(int n1, int n2, int n3, const double normal[3], const int ppoint[3], long long *pts, int cap)
+{
+  /* The predicate of avestrain and Imatrix (G.c:707, 714, 2596), evaluated
+   * in the same order of double operations so that points on the |.| = 1
+   * boundary fall on the same side. */
+  const int k1 = ppoint[0];
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || k1 < 0 || k1 >= n1) return 0;
+  int cnt = 0;
+  for (int k2 = 0; k2 < n2; k2++) {
+    const double d12 = normal[0] * (k1 - ppoint[0]) + normal[1] * (k2 - ppoint[1]);
+    for (int k3 = 0; k3 < n3; k3++) {
+      if (fabs(d12 + normal[2] * (k3 - ppoint[2])) > 1.0) continue;
+      if (pts && cnt < cap) pts[cnt] = ((long long)k1 * n2 + k2) * n3 + k3;
+      cnt++;
+    }
+  }
+  return cnt;
+}

@@ -13,12 +13,16 @@
  * stress1D.dat, strain.txt, stress.txt, xi.dat and xi_bc.dat keep the
  * reference formats.
  *
+ * With energy=1 (the default) it also writes checkEnergy.dat per outer step
+ * and checkEbarrier.dat per GL step from the line sums of Re xi on the
+ * interface line (avestrain G.c:697-718).
+ *
  *   pfdd [preset] [key=value ...]
  * presets: c0 (32^3 x 3, one loop, 500 steps), c1 (128^3 x 12, 16 loops,
  * 1000 steps), c2 (256^3 x 12, 64 loops, 2000 steps, mixed precision).
  * keys: n1 n2 n3 systems(plane111|fcc) nloops rmin rmax seed sigma_amp tau0
  *       steps nt converge plastic_max precision(fp64|mixed) bmode(recompute|stored)
- *       quirk pfout_every fields(1|0) outdir
+ *       quirk pfout_every fields(1|0) energy(1|0) outdir
  */
 #include <math.h>
 #include <stdio.h>
@@ -52,6 +56,7 @@ typedef struct {
   int precision, bmode, quirk;
   int pfout_every;      /* 0: no pfout.dat */
   int fields;           /* 1: strain1D.dat / stress1D.dat per strain call, strain.txt / stress.txt at the end */
+  int energy;           /* 1: checkEnergy.dat per outer step, checkEbarrier.dat per GL step */
   char outdir[512];
 } config;
 
@@ -62,6 +67,7 @@ static void preset(config *c, const char *name)
   c->plastic_max = 3000;
   c->quirk = 1;
   c->fields = 1;
+  c->energy = 1;
   c->precision = PFDD_FP64;
   c->bmode = PFDD_B_RECOMPUTE;
   c->seed = 20261202ULL;
@@ -120,6 +126,7 @@ static void set_key(config *c, const char *kv)
   else if (!strcmp(key, "quirk")) c->quirk = atoi(v);
   else if (!strcmp(key, "pfout_every")) c->pfout_every = atoi(v);
   else if (!strcmp(key, "fields")) c->fields = atoi(v);
+  else if (!strcmp(key, "energy")) c->energy = atoi(v);
   else if (!strcmp(key, "outdir")) snprintf(c->outdir, sizeof(c->outdir), "%s", v);
   else {
     fprintf(stderr, "pfdd: unknown key %s\n", key);
@@ -229,26 +236,67 @@ static int plastic_converge(double gamma, double gammalast, int it_plastic, int 
   return flag;
 }
 
-/* S_ijkl:sigma*mu, the xi-independent part of avepst (avestrain G.c:656-729). */
-static void avepst_uniform(const pfdd_params *p, const double sigma[3][3], double out[9])
+/* The compliance S_ijkl of avestrain (G.c:652-666), cubic form. */
+static double s_ijkl(const pfdd_params *p, int i, int j, int k, int l)
 {
-  double C11 = p->C11, C12 = p->C12, C44 = p->C44, mu = C44;
+  double C11 = p->C11, C12 = p->C12, C44 = p->C44;
   double S11 = 1. / 3. * (1 / (C11 + 2 * C12) + 2 / (C11 - C12));
   double S12 = 1. / 3. * (1 / (C11 + 2 * C12) - 1 / (C11 - C12));
   double S44 = 1 / C44;
+  double dij = i == j, dkl = k == l, dik = i == k, djl = j == l, dil = i == l, djk = j == k;
+  double d4 = (i == j && j == k && k == l);
+  return (S11 + 2 * S12) / 3.0 * dij * dkl + S44 / 4.0 * (dik * djl + dil * djk) - S44 / 2.0 * d4 +
+         (S11 - S12) * (d4 - dij * dkl / 3.0);
+}
+
+/* S_ijkl:sigma*mu, the xi-independent part of avepst (avestrain G.c:720-729). */
+static void avepst_uniform(const pfdd_params *p, const double sigma[3][3], double out[9])
+{
+  const double mu = p->C44;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) {
       double v = 0.0;
       for (int k = 0; k < 3; k++)
-        for (int l = 0; l < 3; l++) {
-          double dij = i == j, dkl = k == l, dik = i == k, djl = j == l, dil = i == l, djk = j == k;
-          double d4 = (i == j && j == k && k == l);
-          double S = (S11 + 2 * S12) / 3.0 * dij * dkl + S44 / 4.0 * (dik * djl + dil * djk) - S44 / 2.0 * d4 +
-                     (S11 - S12) * (d4 - dij * dkl / 3.0);
-          v += S * sigma[k][l] * mu;
-        }
+        for (int l = 0; l < 3; l++) v += s_ijkl(p, i, j, k, l) * sigma[k][l] * mu;
       out[3 * i + j] = v;
     }
+}
+
+/* Energy diagnostics on the interface line (avestrain G.c:697-718 with a
+ * uniform applied stress and no virtual strain): energy_in3 =
+ * -sum_a (e_a : sigma) L_a with L_a the line sum of Re xi_a, and the increment
+ * of energy_in4 per avestrain call, -mu/2 npts sigma:S:sigma.  energy_in
+ * (Energy_calculation) and energy_in2 (Imatrix) read only the virtual fields
+ * and ResidualEnergy needs two materials; all three are 0 here. */
+typedef struct {
+  int npts;
+  double e_sig[PFDD_MAX_NS]; /* e_a : sigma */
+  double e4_inc;
+} line_energy;
+
+static void line_energy_init(line_energy *le, const pfdd_params *p, const double sigma[3][3], int npts)
+{
+  const double mu = p->C44;
+  double ssig = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) ssig += s_ijkl(p, i, j, k, l) * sigma[k][l] * sigma[i][j];
+  le->npts = npts;
+  le->e4_inc = -ssig * 0.5 * mu * npts;
+  for (int a = 0; a < p->ns; a++) {
+    double v = 0.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) v += p->eps[a][i][j] * sigma[i][j];
+    le->e_sig[a] = v;
+  }
+}
+
+static double line_e3(const line_energy *le, int ns, const double *lsum)
+{
+  double e3 = 0.0;
+  for (int a = 0; a < ns; a++) e3 += -le->e_sig[a] * lsum[a];
+  return e3;
 }
 
 int main(int argc, char **argv)
@@ -372,6 +420,30 @@ int main(int argc, char **argv)
   long total_steps = 0;
   double gl_time = 0.0;
 
+  /* interface line of the energy diagnostics: interface_n and ppoint of main (G.c:4200-4203, 4234-4236) */
+  line_energy le;
+  memset(&le, 0, sizeof(le));
+  FILE *ofE = NULL, *ofB = NULL;
+  double *lsum = NULL, e4 = 0.0;
+  int it_checkEbarrier = 0;
+  if (cfg.energy) {
+    const double inorm[3] = {sqrt(3.) / 3., 0.0, sqrt(6.) / 3.};
+    const int pp[3] = {p.n1 / 2, 0, p.n3 * 3 / 4};
+    const int np = pfdd_line_points(p.n1, p.n2, p.n3, inorm, pp, NULL, 0);
+    long long *pts = malloc(sizeof(long long) * (np > 0 ? np : 1));
+    lsum = malloc(sizeof(double) * nsys * (maxsteps > 0 ? maxsteps : 1));
+    if (!pts || !lsum) {
+      fprintf(stderr, "pfdd: host allocation failed\n");
+      return 1;
+    }
+    pfdd_line_points(p.n1, p.n2, p.n3, inorm, pp, pts, np);
+    die(pfdd_set_line_points(ctx, pts, np), "pfdd_set_line_points");
+    free(pts);
+    line_energy_init(&le, &p, (const double(*)[3])sigma, np);
+    ofE = open_out(&cfg, "checkEnergy.dat", "w");
+    ofB = open_out(&cfg, "checkEbarrier.dat", "w");
+  }
+
   printf("start time step \n");
   for (int it = 0; it < cfg.nt; it++) {
     const int itp = 0, itp2 = 0;
@@ -400,6 +472,13 @@ int main(int argc, char **argv)
     }
     /* fft_forward + calc_data2_datag + virtualevolv data2 term */
     die(pfdd_begin_outer(ctx, cfg.quirk), "pfdd_begin_outer");
+    if (cfg.energy) { /* avestrain with energy_in4 reset (G.c:4439-4442), checkEnergy line (G.c:4530-4536) */
+      double ls[PFDD_MAX_NS];
+      die(pfdd_line_sums(ctx, ls), "pfdd_line_sums");
+      const double e3 = line_e3(&le, nsys, ls);
+      e4 = le.e4_inc;
+      fprintf(ofE, "%d    %lf   %lf   %lf   %lf   %lf   %lf\n", it, 0.0, 0.0, e3, e4, 0.0, e3 + e4);
+    }
     printf("vflag %d, time %d \n", 0, it);
     if (it != 0 && it != cfg.nt - 1) {
       printf("here plastic!\n");
@@ -408,11 +487,17 @@ int main(int argc, char **argv)
       memset(testplastic, 0, sizeof(int) * (cfg.plastic_max > 0 ? cfg.plastic_max : 1));
       double t0 = wtime();
       if (!cfg.converge) {
-        die(pfdd_gl_steps(ctx, cfg.steps, gam, gam_a), "pfdd_gl_steps");
+        if (cfg.energy) die(pfdd_gl_steps_lines(ctx, cfg.steps, gam, gam_a, lsum), "pfdd_gl_steps_lines");
+        else die(pfdd_gl_steps(ctx, cfg.steps, gam, gam_a), "pfdd_gl_steps");
         gl_time += wtime() - t0;
         for (int s = 0; s < cfg.steps; s++) {
           gamma = gam[s];
           printf("in evolve plastic:    %d    %d\n", s, 1);
+          if (cfg.energy) { /* G.c:4646-4650 after every GL step */
+            const double e3 = line_e3(&le, nsys, lsum + (long)s * nsys);
+            e4 += le.e4_inc;
+            fprintf(ofB, "%d   %lf   %lf\n", it_checkEbarrier++, 0.0, fabs(e3 + e4));
+          }
           fprintf(ofpfv, "%d       %d         %e           %e           %e\n", it, s + 1, gam_a[(long)s * nsys],
                   nsys > 1 ? gam_a[(long)s * nsys + 1] : 0.0, gamma);
         }
@@ -421,11 +506,17 @@ int main(int argc, char **argv)
       } else {
         do { /* G.c:4574-4691, one GL step per call */
           gammalast = gamma;
-          die(pfdd_gl_steps(ctx, 1, gam, gam_a), "pfdd_gl_steps");
+          if (cfg.energy) die(pfdd_gl_steps_lines(ctx, 1, gam, gam_a, lsum), "pfdd_gl_steps_lines");
+          else die(pfdd_gl_steps(ctx, 1, gam, gam_a), "pfdd_gl_steps");
           gamma = gam[0];
           for (int a = 0; a < nsys; a++) gamma1[a] = gam_a[a];
           check = plastic_converge(gamma, gammalast, it_plastic, testplastic, &countgamma);
           printf("in evolve plastic:    %d    %d\n", it_plastic, check);
+          if (cfg.energy && check) { /* only while plasticconverge says go on (G.c:4589, 4646-4650) */
+            const double e3 = line_e3(&le, nsys, lsum);
+            e4 += le.e4_inc;
+            fprintf(ofB, "%d   %lf   %lf\n", it_checkEbarrier++, 0.0, fabs(e3 + e4));
+          }
           it_plastic++;
           total_steps++;
           fprintf(ofpfv, "%d       %d         %e           %e           %e\n", it, it_plastic, gamma1[0],
@@ -458,6 +549,9 @@ int main(int argc, char **argv)
   if (ofpfout) fclose(ofpfout);
   if (of4) fclose(of4);
   if (of6) fclose(of6);
+  if (ofE) fclose(ofE);
+  if (ofB) fclose(ofB);
+  free(lsum);
   free(efield);
   free(sfield);
   pfdd_destroy(ctx);

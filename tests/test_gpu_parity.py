@@ -344,6 +344,97 @@ def test_strain_stress_vs_oracle():
 
 
 # ---------------------------------------------------------------- boundary error behaviour
+def _s_consts():
+    S11 = 1. / 3. * (1 / (C.C11 + 2 * C.C12) + 2 / (C.C11 - C.C12))
+    S12 = 1. / 3. * (1 / (C.C11 + 2 * C.C12) - 1 / (C.C11 - C.C12))
+    return S11, S12, 1 / C.C44
+
+
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+def test_line_sums_and_energy_vs_oracle(prec):
+    """Line sums of Re xi on the interface line of avestrain (G.c:697-718)
+    through pfdd_line_sums and pfdd_gl_steps_lines against the oracle's xi
+    after every step; the energies built from them against the oracle's line
+    energy (which the CPU tests pin to the compiled reference's avestrain);
+    the line kernel leaves the trajectory bit-identical."""
+    import dataclasses
+    say(f"line sums prec={prec}")
+    w0 = C.small_fcc((32, 32, 32))
+    rng = np.random.default_rng(5)
+    w = dataclasses.replace(w0, xi=w0.xi + 0.2 * rng.random(w0.xi.shape))   # non-zero on the line
+    pts = C.line_points(w.shape)
+    assert np.array_equal(pts, O.line_points(w.shape, C.INTERFACE_N, C.interface_ppoint(w.shape)))
+    steps = 20
+    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * w.ns, C.asf2_reference(w.ns), [1.0] * w.ns)
+    o = O.Oracle(p)
+    o.set_xi(w.xi)
+    o.set_tau(w.tau)
+    o.prologue(True)
+    want0 = w.xi.reshape(w.ns, -1).real[:, pts].sum(1)
+    want = []
+    for _ in range(steps):
+        o.gl_steps(1)
+        want.append(o.get_xi().reshape(w.ns, -1).real[:, pts].sum(1))
+    want = np.array(want)
+    with pa.Context(w.params(prec)) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi)
+        c.set_line_points(pts)
+        l0 = c.line_sums()
+        assert c.step_plan()[pa.STAGE_FINISH] == 2      # the step's reduction + the line sums
+        c.begin_outer(True)
+        g, ga, ln = c.gl_steps_lines(steps)
+        xend = c.download_xi()
+    scale = np.abs(want).max()
+    assert scale > 1.0
+    assert np.abs(l0 - want0).max() <= 1e-12 * scale
+    got_end = xend.reshape(w.ns, -1).real[:, pts].sum(1)
+    assert np.abs(ln[-1] - got_end).max() <= 1e-12 * scale
+    tol = 1e-10 if prec == pa.FP64 else 1e-4
+    assert np.abs(ln - want).max() <= tol * scale
+    xg, gg, gag = run_gpu(w, steps, precision=prec)
+    assert np.array_equal(xg, xend) and np.array_equal(gg, g) and np.array_equal(gag, ga)
+    # energy_in3 = -sum_a (e_a : sigma) L_a and the energy_in4 increment, against the oracle's avestrain
+    S11, S12, S44 = _s_consts()
+    e3_o, e4_o = o.line_energy(w.sigma, S11, S12, S44, C.C44, pts)
+    e3 = -sum(np.sum(w.eps[a] * w.sigma) * ln[-1][a] for a in range(w.ns))
+    assert abs(e3 - e3_o) <= tol * max(1.0, abs(e3_o)) and abs(e3_o) > 1e-3
+    d = np.eye(3)
+    d4 = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        d4[i, i, i, i] = 1.0
+    Sm = ((S11 + 2 * S12) / 3.0 * np.einsum("ij,kl->ijkl", d, d)
+          + S44 / 4.0 * (np.einsum("ik,jl->ijkl", d, d) + np.einsum("il,jk->ijkl", d, d)) - S44 / 2.0 * d4
+          + (S11 - S12) * (d4 - np.einsum("ij,kl->ijkl", d, d) / 3.0))
+    e4 = -0.5 * C.C44 * len(pts) * np.einsum("ijkl,kl,ij->", Sm, w.sigma, w.sigma)
+    assert abs(e4 - e4_o) <= 1e-12 * abs(e4_o)
+
+
+def test_line_sums_across_history_chunks():
+    """More steps than one history chunk (1024 rows): every row is written,
+    the rows of a shorter run agree bit for bit, and the last row equals the
+    sums of the downloaded xi."""
+    say("line sums over 1030 steps")
+    w = C.small_fcc((16, 16, 16))
+    pts = C.line_points(w.shape)
+    with pa.Context(w.params()) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi + 0.1)
+        c.set_line_points(pts)
+        c.begin_outer(True)
+        g, _, ln = c.gl_steps_lines(1030)
+        xend = c.download_xi()
+        c.upload_xi(w.xi + 0.1)
+        c.begin_outer(True)
+        _, _, ln20 = c.gl_steps_lines(20)
+        c.set_line_points([])
+        with pytest.raises(pa.PfddError):
+            c.gl_steps_lines(1)
+    assert np.array_equal(ln[:20], ln20)
+    assert np.abs(ln[-1] - xend.reshape(w.ns, -1).real[:, pts].sum(1)).max() <= 1e-12 * np.abs(ln[-1]).max()
+    assert np.all(np.any(ln[1:] != ln[:-1], axis=1))
+
+
 def test_error_codes():
     say("error codes")
     w = C.small_fcc((8, 8, 8), ns=2)
@@ -407,7 +498,14 @@ def test_driver_c0_matches_oracle(tmp_path):
     o.set_xi(w.xi)
     o.set_tau(w.tau)
     o.prologue(True)
-    o.gl_steps(40)
+    pts = O.line_points(w.shape, C.INTERFACE_N, C.interface_ppoint(w.shape))
+    S11, S12, S44 = _s_consts()
+    e3_0, inc = o.line_energy(w.sigma, S11, S12, S44, C.C44, pts)
+    bar = []
+    for s in range(40):
+        o.gl_steps(1)
+        e3_end, _ = o.line_energy(w.sigma, S11, S12, S44, C.C44, pts)
+        bar.append(abs(e3_end + (s + 2) * inc))    # energy_in4 gains one increment per avestrain call
     _, _, ef, sf = o.strain_stress(w.s_tensor_avepst(), fields=True)
     n1, n2, n3 = w.shape
     st = np.loadtxt(tmp_path / "strain.txt", delimiter=",", skiprows=1)
@@ -425,6 +523,15 @@ def test_driver_c0_matches_oracle(tmp_path):
         last = np.array([[float(v) for v in l.split()] for l in zones[-1].splitlines()[1:] if l.strip()])
         assert np.array_equal(last[:, 0], np.arange(n1))
         assert np.abs(last[:, 1] - scale * f[2, 2, :, n2 // 2, n3 // 2]).max() <= 6e-7
+    # energy diagnostics: checkEnergy.dat per outer step (G.c:4535), checkEbarrier.dat per GL step (G.c:4650)
+    E = np.loadtxt(tmp_path / "checkEnergy.dat")
+    assert E.shape == (3, 7) and np.array_equal(E[:, 0], [0, 1, 2]) and np.all(E[:, [1, 2, 5]] == 0)
+    assert np.abs(E[:2, 3] - e3_0).max() <= 6e-7 and abs(E[2, 3] - e3_end) <= 6e-7
+    assert np.abs(E[:, 4] - inc).max() <= 6e-7 and abs(inc) > 1e-4
+    assert np.abs(E[:, 6] - E[:, 3] - E[:, 4]).max() <= 2e-6
+    Bt = np.loadtxt(tmp_path / "checkEbarrier.dat")
+    assert Bt.shape == (40, 3) and np.array_equal(Bt[:, 0], np.arange(40)) and np.all(Bt[:, 1] == 0)
+    assert np.abs(Bt[:, 2] - np.array(bar)).max() <= 6e-7
 
 
 def test_driver_converge_mode_runs():
@@ -439,6 +546,9 @@ def test_driver_converge_mode_runs():
         n = len(np.loadtxt(os.path.join(d, "phasefield.dat")).reshape(-1, 5))
         assert 1 <= n <= 120
         assert "zone   I = 32 K = 32" in open(os.path.join(d, "pfout.dat")).read()
+        # a checkEbarrier line only for the steps after which plasticconverge says go on
+        nb = len(np.loadtxt(os.path.join(d, "checkEbarrier.dat")).reshape(-1, 3))
+        assert nb == n or (nb == n - 1 and n < 120)
 
 
 def test_fft_host_seam_matches_reference_fourn():

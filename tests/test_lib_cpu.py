@@ -63,6 +63,29 @@ def test_create_validates_before_touching_the_device():
     assert _create_rc(p) == pa.E_SHAPE
 
 
+@pytest.mark.parametrize("shape,pp", [((32, 2, 32), None), ((128, 128, 128), None), ((16, 12, 8), (3, 5, 6)),
+                                       ((8, 8, 8), (0, 0, 0)), ((8, 8, 8), (9, 0, 0))])
+def test_line_points_match_brute_force(shape, pp):
+    """pfdd_line_points against the predicate of avestrain (G.c:707) by brute force."""
+    ppoint = C.interface_ppoint(shape) if pp is None else pp
+    n = C.INTERFACE_N
+    want = [(k1 * shape[1] + k2) * shape[2] + k3
+            for k1 in range(shape[0]) for k2 in range(shape[1]) for k3 in range(shape[2])
+            if k1 == ppoint[0] and abs(n[0] * (k1 - ppoint[0]) + n[1] * (k2 - ppoint[1]) + n[2] * (k3 - ppoint[2])) <= 1.0]
+    got = C.line_points(shape, ppoint=ppoint)
+    assert got.tolist() == want
+    if shape == (128, 128, 128):
+        # |sqrt(6)/3 (k3 - 96)| <= 1 holds for k3 = 95, 96, 97 on every k2
+        assert len(got) == 3 * 128
+
+
+def test_line_point_calls_validate_before_touching_the_device():
+    L = pa.load()
+    assert L.pfdd_set_line_points(None, None, 0) == pa.E_SHAPE
+    assert L.pfdd_line_sums(None, None) == pa.E_SHAPE
+    assert L.pfdd_gl_steps_lines(None, 1, None, None, None) == pa.E_SHAPE
+
+
 def test_splitmix64_known_answers():
     """First outputs of SplitMix64 for seed 1234567 (Vigna's reference implementation)."""
     r = C.Rng(1234567)
