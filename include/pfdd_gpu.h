@@ -81,7 +81,15 @@ enum {
    * of radix-8 first (more passes, twice the threads per pencil) */
   PFDD_PLAN_RADIX4_AXIS1 = 1 << 16,
   PFDD_PLAN_RADIX4_AXIS2 = 1 << 17,
-  PFDD_PLAN_RADIX4_AXIS3 = 1 << 18
+  PFDD_PLAN_RADIX4_AXIS3 = 1 << 18,
+  /* layout of the FFT work array.  INTERLEAVED keeps the ns fields of a grid
+   * point together ([instance][N1][N2][N3][ns]), so that the fused row and
+   * column kernels read and write all fields of their rows / columns as one
+   * contiguous stretch; it needs the fused row kernel and excludes the plane
+   * kernel, and the row / column tile bits then count rows / columns of all
+   * fields.  FIELD_MAJOR ([instance][ns][N1][N2][N3]) forces the other. */
+  PFDD_PLAN_INTERLEAVED = 1 << 19,
+  PFDD_PLAN_FIELD_MAJOR = 1 << 20
 };
 #define PFDD_PLAN_ROW_TILE_LOG2(l) (((l) + 1) << PFDD_PLAN_ROW_TILE_SHIFT)
 #define PFDD_PLAN_COL_TILE_LOG2(l) (((l) + 1) << PFDD_PLAN_COL_TILE_SHIFT)

@@ -21,6 +21,7 @@ FORWARD, BACKWARD = -1, 1
 PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS, PLAN_NO_PLANE, PLAN_PLANE = 0, 1, 2, 4, 8, 16
 PLAN_ROW_TILE_SHIFT, PLAN_COL_TILE_SHIFT = 8, 12
 PLAN_RADIX4_AXIS1, PLAN_RADIX4_AXIS2, PLAN_RADIX4_AXIS3 = 1 << 16, 1 << 17, 1 << 18
+PLAN_INTERLEAVED, PLAN_FIELD_MAJOR = 1 << 19, 1 << 20
 
 
 def plan_row_tile(tile):
@@ -36,7 +37,7 @@ def plan_col_tile(tile):
 def describe_plan(plan):
     names = [n for n, v in (("unfused_rows", 1), ("unfused_cols", 2), ("fused_cols", 4), ("no_plane", 8),
                             ("plane", 16), ("radix4_n1", 1 << 16), ("radix4_n2", 1 << 17),
-                            ("radix4_n3", 1 << 18)) if plan & v]
+                            ("radix4_n3", 1 << 18), ("interleaved", 1 << 19), ("field_major", 1 << 20)) if plan & v]
     rt, ct = (plan >> PLAN_ROW_TILE_SHIFT) & 15, (plan >> PLAN_COL_TILE_SHIFT) & 15
     if rt:
         names.append(f"row_tile={1 << (rt - 1)}")

@@ -82,6 +82,7 @@ struct pfdd_ctx {
   int row_nblk = 0;          // its blocks per field
   int c3_tile = 0;           // columns per block of the fused column kernel (0: not fused)
   int plane = 0;             // 1: the LDS-resident plane kernel does the N2/N3 passes and the update
+  int il = 0;                // 1: W interleaved, [B][P][S] (fields of a point together); else [B][S][P]
   size_t c3_lds = 0;         // its dynamic LDS bytes
   int plan = 0;              // the applied PFDD_PLAN_* word
   int r4 = 0;                // bit ax: radix-4 sequence for the standalone passes along axis ax
@@ -213,13 +214,20 @@ int contract(pfdd_ctx* c, int quirk) {
   const dim3 grid((unsigned)((c->P + 255) / 256), (unsigned)c->B);
   cx<T>* W = (cx<T>*)c->W;
   const T* B = (const T*)c->Bsym;
-#define PFDD_CT(SM)                                                                                     \
-  if (c->stored)                                                                                        \
-    hipLaunchKernelGGL((k_contract<T, SM, true>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
-  else                                                                                                  \
-    hipLaunchKernelGGL((k_contract<T, SM, false>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
+#define PFDD_CT1(SM, IL)                                                                                       \
+  if (c->stored)                                                                                               \
+    hipLaunchKernelGGL((k_contract<T, SM, true, IL>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_contract<T, SM, false, IL>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
+#define PFDD_CT(SM)     \
+  if (c->il) {          \
+    PFDD_CT1(SM, true)  \
+  } else {              \
+    PFDD_CT1(SM, false) \
+  }
   if (c->S <= 4) { PFDD_CT(4) } else if (c->S <= 8) { PFDD_CT(8) } else if (c->S <= 12) { PFDD_CT(12) } else { PFDD_CT(16) }
 #undef PFDD_CT
+#undef PFDD_CT1
   tap(c, PFDD_STAGE_CONTRACT);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
@@ -286,20 +294,89 @@ void k1_launch_n(pfdd_ctx* c) {
 #undef PFDD_K1
 }
 
+// Threads of an interleaved-layout block: one TPP-lane group per pencil, in
+// whole waves.
+template <int N> int il_threads(int npen) { return ((npen * make_plan(N).tpp + 63) / 64) * 64; }
+
+// Row kernel, interleaved layout (c->row_tile rows of every field per block);
+// attr_only sets its dynamic-LDS limit instead of launching.
+template <typename T, int N>
+int k1il_launch_n(pfdd_ctx* c, bool attr_only) {
+  const int nt = il_threads<N>(c->S * c->row_tile);
+  const size_t lds = row_il_lds_bytes<T, N>(c->S, c->row_tile, nt);
+  cx<T>* W = (cx<T>*)c->W;
+  const cx<T>* tw = tw_of<T>(c->tw[2]);
+#define PFDD_K1(MM, TT)                                                                                           \
+  if (attr_only) {                                                                                                \
+    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step_il<T, N, MM, TT>),                        \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                            \
+  } else {                                                                                                        \
+    hipLaunchKernelGGL((k_row_step_il<T, N, MM, TT>), dim3((unsigned)c->row_nblk, (unsigned)c->B), dim3(nt), lds, \
+                       c->stream, W, xm<T>(c), c->mask, c->taupp, c->sa, c->row_tile, tw, c->partials,            \
+                       c->step_ctr);                                                                              \
+  }
+  if (attr_only) {
+    PFDD_K1(true, true) PFDD_K1(true, false) PFDD_K1(false, true) PFDD_K1(false, false)
+  } else {
+    const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+    if (m && tp) { PFDD_K1(true, true) }
+    else if (m) { PFDD_K1(true, false) }
+    else if (tp) { PFDD_K1(false, true) }
+    else { PFDD_K1(false, false) }
+  }
+#undef PFDD_K1
+  return PFDD_OK;
+}
+
+template <typename T>
+int k1il_attr(pfdd_ctx* c) {
+  switch (c->p.n3) {
+#define PFDD_L(NN) case NN: return k1il_launch_n<T, NN>(c, true);
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+#undef PFDD_L
+    default: return PFDD_E_RADIX;
+  }
+}
+
+// The interleaved row kernel writes one block sum per field and row block;
+// above this many per field they are folded to FOLD first.
+constexpr int FOLD_ABOVE = 2048;
+bool folds(const pfdd_ctx* c) { return c->il && c->row_nblk > FOLD_ABOVE; }
+
+// The gamma sums of the fused row kernel's block sums (k_finish_sums, with
+// step_ctr or without it).
+void finish_rows(pfdd_ctx* c, int* step_ctr) {
+  const double* src = c->partials;
+  int nblk = c->row_nblk;
+  if (folds(c)) {
+    double* dst = c->partials + c->partials_len - (size_t)c->F * FOLD;
+    hipLaunchKernelGGL(k_fold_partials, dim3(FOLD, (unsigned)c->F), dim3(256), 0, c->stream, c->partials,
+                       c->row_nblk, dst);
+    tap(c, PFDD_STAGE_FINISH);
+    src = dst;
+    nblk = FOLD;
+  }
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, src, c->F, nblk, c->S, c->hist,
+                     step_ctr);
+  tap(c, PFDD_STAGE_FINISH);
+}
+
 template <typename T>
 int k1_launch(pfdd_ctx* c, bool finish = true) {
   switch (c->p.n3) {
-#define PFDD_L(NN) case NN: k1_launch_n<T, NN>(c); break;
+#define PFDD_L(NN)                    \
+  case NN:                            \
+    if (c->il)                        \
+      k1il_launch_n<T, NN>(c, false); \
+    else                              \
+      k1_launch_n<T, NN>(c);          \
+    break;
     PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
 #undef PFDD_L
     default: return PFDD_E_RADIX;
   }
   tap(c, PFDD_STAGE_UPDATE);
-  if (finish) {
-    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F, c->row_nblk, c->S,
-                       c->hist, c->step_ctr);
-    tap(c, PFDD_STAGE_FINISH);
-  }
+  if (finish) finish_rows(c, c->step_ctr);
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -326,10 +403,33 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
+// Column kernel, interleaved layout (c->c3_tile columns of every field per block).
+template <typename T, int N>
+int k3il_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
+  const long nb = ((long)c->p.n2 * c->p.n3) / c->c3_tile;
+  const int nt = il_threads<N>(c->S * c->c3_tile);
+  const size_t lds = col_il_lds_bytes<T, N>(c->S, c->c3_tile);
+  cx<T>* W = (cx<T>*)c->W;
+  const cx<T>* tw = tw_of<T>(c->tw[0]);
+#define PFDD_K3(SM)                                                                                            \
+  if (attr_only) {                                                                                             \
+    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_col_contract_il<T, N, SM>),                     \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                         \
+  } else {                                                                                                     \
+    hipLaunchKernelGGL((k_col_contract_il<T, N, SM>), dim3((unsigned)nb, (unsigned)c->B), dim3(nt), lds,        \
+                       c->stream, W, c->ec, c->P, c->c3_tile, quirk, tw);                                      \
+  }
+  if (c->S <= 4) { PFDD_K3(4) } else if (c->S <= 8) { PFDD_K3(8) } else if (c->S <= 12) { PFDD_K3(12) } else { PFDD_K3(16) }
+#undef PFDD_K3
+  if (!attr_only) tap(c, PFDD_STAGE_CONTRACT);
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
 template <typename T>
 int k3_launch(pfdd_ctx* c, int quirk, bool attr_only = false) {
   switch (c->p.n1) {
-#define PFDD_L(NN) case NN: return k3_launch_n<T, NN>(c, quirk, attr_only);
+#define PFDD_L(NN) \
+  case NN: return c->il ? k3il_launch_n<T, NN>(c, quirk, attr_only) : k3_launch_n<T, NN>(c, quirk, attr_only);
     PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
 #undef PFDD_L
     default: return PFDD_E_RADIX;
@@ -398,19 +498,34 @@ int plane_launch(pfdd_ctx* c, bool finish = true, bool attr_only = false) {
 
 // The N1 part of the elastic solve: forward N1, contraction, inverse N1
 // (fused into one kernel when it fits).
+// One FFT pass of W along axis ax (0: N1, 1: N2, 2: N3) for the context's
+// layout: W is [outer][n][inner] with the fields in `outer` (field-major)
+// or innermost (interleaved).
+template <typename T>
+int pass(pfdd_ctx* c, int ax, int sign) {
+  const long n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3;
+  const long fo = c->il ? c->B : c->F, fi = c->il ? c->S : 1;
+  long outer, inner;
+  int n;
+  if (ax == 2) {
+    outer = fo * n1 * n2, n = (int)n3, inner = fi;
+  } else if (ax == 1) {
+    outer = fo * n1, n = (int)n2, inner = n3 * fi;
+  } else {
+    outer = fo, n = (int)n1, inner = n2 * n3 * fi;
+  }
+  const int rc = launch_axis<T>((cx<T>*)c->W, outer, n, inner, sign, tw_of<T>(c->tw[ax]), c->stream, rl_of(c, ax));
+  tap(c, ax == 0 ? PFDD_STAGE_FFT_AXIS1 : ax == 1 ? PFDD_STAGE_FFT_AXIS2 : PFDD_STAGE_FFT_AXIS3);
+  return rc;
+}
+
 template <typename T>
 int n1_part(pfdd_ctx* c, int quirk) {
   int rc;
-  cx<T>* W = (cx<T>*)c->W;
-  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
   if (c->c3_tile > 0) return k3_launch<T>(c, quirk);
-  if ((rc = launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_FORWARD,
-                           tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0)))) return rc;
-  tap(c, PFDD_STAGE_FFT_AXIS1);
+  if ((rc = pass<T>(c, 0, PFDD_FORWARD))) return rc;
   if ((rc = contract<T>(c, quirk))) return rc;
-  rc = launch_axis<T>(W, F, n1, (long)n2 * n3, PFDD_BACKWARD, tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0));
-  tap(c, PFDD_STAGE_FFT_AXIS1);
-  return rc;
+  return pass<T>(c, 0, PFDD_BACKWARD);
 }
 
 // Passes between the forward N3 pass and the inverse N3 pass: forward N2,
@@ -419,23 +534,14 @@ int n1_part(pfdd_ctx* c, int quirk) {
 template <typename T>
 int middle(pfdd_ctx* c, int quirk) {
   int rc;
-  cx<T>* W = (cx<T>*)c->W;
-  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
-  if ((rc = launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_FORWARD,
-                           tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1)))) return rc;
-  tap(c, PFDD_STAGE_FFT_AXIS2);
+  if ((rc = pass<T>(c, 1, PFDD_FORWARD))) return rc;
   if ((rc = n1_part<T>(c, quirk))) return rc;
-  rc = launch_axis<T>(W, (long)F * n1, n2, n3, PFDD_BACKWARD, tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1));
-  tap(c, PFDD_STAGE_FFT_AXIS2);
-  return rc;
+  return pass<T>(c, 1, PFDD_BACKWARD);
 }
 
 template <typename T>
 int axis3(pfdd_ctx* c, int sign) {
-  const int rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1 * c->p.n2, c->p.n3, 1, sign,
-                           tw_of<T>(c->tw[2]), c->stream, rl_of(c, 2));
-  tap(c, PFDD_STAGE_FFT_AXIS3);
-  return rc;
+  return pass<T>(c, 2, sign);
 }
 
 // Between GL steps W holds data2 with the inverse transform done along N1
@@ -463,12 +569,15 @@ template <typename T>
 int prologue(pfdd_ctx* c, int quirk) {
   int rc;
   const long n = (long)c->F * c->P;
-  hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c), (cx<T>*)c->W, n);
+  if (c->il)
+    hipLaunchKernelGGL((k_xi_to_w_il<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c),
+                       (cx<T>*)c->W, c->P, c->S, n);
+  else
+    hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c),
+                       (cx<T>*)c->W, n);
   if ((rc = axis3<T>(c, PFDD_FORWARD))) return rc;
   if (c->plane) {  // the next step's plane kernel finishes the N2 and N3 inverse passes
-    if ((rc = launch_axis<T>((cx<T>*)c->W, (long)c->F * c->p.n1, c->p.n2, c->p.n3, PFDD_FORWARD,
-                             tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1))))
-      return rc;
+    if ((rc = pass<T>(c, 1, PFDD_FORWARD))) return rc;
     return n1_part<T>(c, quirk);
   }
   if ((rc = middle<T>(c, quirk))) return rc;
@@ -647,7 +756,51 @@ void pfdd_destroy(pfdd_ctx* c) {
 namespace {
 
 constexpr int PLAN_FLAGS = PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_FUSED_COLS |
-                           PFDD_PLAN_NO_PLANE | PFDD_PLAN_PLANE;
+                           PFDD_PLAN_NO_PLANE | PFDD_PLAN_PLANE | PFDD_PLAN_INTERLEAVED | PFDD_PLAN_FIELD_MAJOR;
+// LDS of one workgroup on gfx950
+constexpr size_t LDS_MAX = 160 * 1024;
+// AUTO tiles of the interleaved kernels stay within this much LDS (two or
+// more blocks per CU)
+constexpr size_t IL_AUTO_LDS = 64 * 1024;
+
+// Rows (per block) of the interleaved row kernel and columns of the
+// interleaved column kernel: the requested ones (log2 + 1 in the plan word,
+// PFDD_E_SHAPE when they do not fit) or, for AUTO, one row, and the widest
+// column tile whose rows of all fields stay within 256 bytes.
+int il_tiles(pfdd_ctx* c, int plan, int& row_tile, int& c3_tile, size_t& c3_lds) {
+  const pfdd_params& p = c->p;
+  const int S = c->S;
+  const size_t csz = c->fp64 ? sizeof(double2) : sizeof(float2);
+  const int rreq = (plan >> PFDD_PLAN_ROW_TILE_SHIFT) & 15, creq = (plan >> PFDD_PLAN_COL_TILE_SHIFT) & 15;
+  const long rows = (long)p.n1 * p.n2, inner = (long)p.n2 * p.n3;
+  const int tpp3 = make_plan(p.n3).tpp, tpp1 = make_plan(p.n1).tpp;
+  auto nt = [](int npen, int tpp) { return ((npen * tpp + 63) / 64) * 64; };
+  auto row_lds = [&](int r) { return (size_t)S * r * (p.n3 + 1) * csz + (size_t)nt(S * r, tpp3) * sizeof(double); };
+  auto row_fits = [&](int r) { return nt(S * r, tpp3) <= 1024 && rows % r == 0 && row_lds(r) <= LDS_MAX; };
+  row_tile = rreq ? 1 << (rreq - 1) : 1;
+  if (!row_fits(row_tile)) return PFDD_E_SHAPE;
+  c3_tile = 0;
+  c3_lds = 0;
+  if (fused_len(p.n1) && !c->stored && !(plan & PFDD_PLAN_UNFUSED_COLS)) {
+    auto col_lds = [&](int t) { return (size_t)S * t * (p.n1 + 1) * csz; };
+    auto col_fits = [&](int t) { return nt(S * t, tpp1) <= 1024 && inner % t == 0 && col_lds(t) <= LDS_MAX; };
+    int t = 1;
+    if (creq) {
+      t = 1 << (creq - 1);
+      if (!col_fits(t)) return PFDD_E_SHAPE;
+    } else {
+      while (col_fits(t * 2) && col_lds(t * 2) <= IL_AUTO_LDS && (size_t)t * 2 * S * csz <= 256) t *= 2;
+      if (!col_fits(t)) t = 0;
+    }
+    if (t > 0) {
+      c3_tile = t;
+      c3_lds = col_lds(t);
+    }
+  } else if (creq) {
+    return PFDD_E_SHAPE;
+  }
+  return PFDD_OK;
+}
 constexpr int PLAN_TILES = (15 << PFDD_PLAN_ROW_TILE_SHIFT) | (15 << PFDD_PLAN_COL_TILE_SHIFT) |
                            PFDD_PLAN_RADIX4_AXIS1 | PFDD_PLAN_RADIX4_AXIS2 | PFDD_PLAN_RADIX4_AXIS3;
 
@@ -667,11 +820,19 @@ int apply_plan(pfdd_ctx* c, int plan) {
   const int S = c->S;
   const size_t csz = c->fp64 ? sizeof(double2) : sizeof(float2);
   const int rreq = (plan >> PFDD_PLAN_ROW_TILE_SHIFT) & 15, creq = (plan >> PFDD_PLAN_COL_TILE_SHIFT) & 15;
-  int row_tile = 0, row_nblk = 0, c3_tile = 0, plane = 0;
+  int row_tile = 0, row_nblk = 0, c3_tile = 0, plane = 0, il = 0;
   size_t c3_lds = 0;
-  // fused row kernel: rows per block = the largest power of two <= its
-  // compile-time tile that divides the N1*N2 rows of a field (or the one asked for)
-  if (fused_len(p.n3) && !(plan & PFDD_PLAN_UNFUSED_ROWS)) {
+  if ((plan & PFDD_PLAN_INTERLEAVED) && (plan & PFDD_PLAN_FIELD_MAJOR)) return PFDD_E_SHAPE;
+  if (plan & PFDD_PLAN_INTERLEAVED) {
+    // interleaved layout: the interleaved row kernel, no plane kernel
+    if (!fused_len(p.n3) || (plan & (PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_PLANE))) return PFDD_E_SHAPE;
+    const int rc = il_tiles(c, plan, row_tile, c3_tile, c3_lds);
+    if (rc) return rc;
+    row_nblk = (int)((long)p.n1 * p.n2 / row_tile);
+    il = 1;
+  } else if (fused_len(p.n3) && !(plan & PFDD_PLAN_UNFUSED_ROWS)) {
+    // fused row kernel: rows per block = the largest power of two <= its
+    // compile-time tile that divides the N1*N2 rows of a field (or the one asked for)
     const long rows = (long)p.n1 * p.n2;
     const int tmax = row_tile_max(p.n3, c->fp64);
     int t = 1;
@@ -687,7 +848,9 @@ int apply_plan(pfdd_ctx* c, int plan) {
     return PFDD_E_SHAPE;
   }
   // fused column kernel: S*tile pencils of N1 points in LDS
-  if (fused_len(p.n1) && !c->stored && !(plan & PFDD_PLAN_UNFUSED_COLS)) {
+  if (il) {
+    // (tiles chosen above)
+  } else if (fused_len(p.n1) && !c->stored && !(plan & PFDD_PLAN_UNFUSED_COLS)) {
     const size_t per_col = (size_t)S * (p.n1 + 1) * csz;
     const int maxpen = (c->fp64 ? 1 : 2) * (C3_THREADS / make_plan(p.n1).tpp);
     const long inner = (long)p.n2 * p.n3;
@@ -709,7 +872,7 @@ int apply_plan(pfdd_ctx* c, int plan) {
   }
   // the plane kernel holds one block per CU; at 128^2 (FP32) it measured no faster than the row kernel +
   // two N2 passes (0.36 vs 0.35 ms at 128^3 x 12), so AUTO takes it up to 96^2 and PFDD_PLAN_PLANE forces it
-  if (!(plan & PFDD_PLAN_NO_PLANE) && p.n2 == p.n3 && plane_len(p.n3, c->fp64) &&
+  if (!il && !(plan & PFDD_PLAN_NO_PLANE) && p.n2 == p.n3 && plane_len(p.n3, c->fp64) &&
       (p.n3 <= 96 || (plan & PFDD_PLAN_PLANE)))
     plane = 1;
   c->r4 = ((plan & PFDD_PLAN_RADIX4_AXIS1) ? 1 : 0) | ((plan & PFDD_PLAN_RADIX4_AXIS2) ? 2 : 0) |
@@ -719,7 +882,9 @@ int apply_plan(pfdd_ctx* c, int plan) {
   c->c3_tile = c3_tile;
   c->c3_lds = c3_lds;
   c->plane = plane;
+  c->il = il;
   int rc = PFDD_OK;
+  if (il && (rc = c->fp64 ? k1il_attr<double>(c) : k1il_attr<float>(c))) return rc;
   if (c3_tile > 0 && (rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true))) return rc;
   if (plane && (rc = c->fp64 ? plane_launch<double>(c, false, true) : plane_launch<float>(c, false, true))) return rc;
   drop_graph(c);
@@ -792,8 +957,10 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     c->upd_nblk = (int)((P + UPD_THREADS * UPD_EPT - 1) / (UPD_THREADS * UPD_EPT));
     const long strain_nblk = (P + 255) / 256;
     // block sums of every plan (the row kernel's tile may go down to one row)
+    // (+ the folded sums of the interleaved row kernel, at the end)
     c->partials_len = (size_t)std::max<long>(std::max<long>((long)F * c->upd_nblk, 18 * strain_nblk),
-                                             std::max<long>((long)F * p->n1 * p->n2, (long)F * p->n1));
+                                             std::max<long>((long)F * p->n1 * p->n2, (long)F * p->n1)) +
+                      (size_t)F * FOLD;
     c->xi_bytes = (c->fp64 ? master_bytes<double>() : master_bytes<float>()) * F * P;
     if (!(c->xi = dev_alloc<char>(c->xi_bytes, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->W = dev_alloc<char>(csz * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -1138,7 +1305,7 @@ std::vector<int> plan_candidates(pfdd_ctx* c) {
   const int saved = c->plan;
   auto add = [&](int plan) {
     if (apply_plan(c, plan) != PFDD_OK) return;
-    const long sig = ((long)c->row_tile << 32) | ((long)c->c3_tile << 8) | c->plane;
+    const long sig = ((long)c->row_tile << 32) | ((long)c->c3_tile << 8) | (c->il << 1) | c->plane;
     for (long v : seen)
       if (v == sig) return;
     seen.push_back(sig);
@@ -1155,6 +1322,20 @@ std::vector<int> plan_candidates(pfdd_ctx* c) {
     if (ct > 0)
       for (lg = 0; (1 << lg) < ct; lg++) {}
     for (int d = 1; d <= 2 && ct > 0 && lg - d >= 0; d++) add(b | ((lg - d + 1) << PFDD_PLAN_COL_TILE_SHIFT));
+  }
+  // interleaved layout: 1, 2 or 4 rows per row block; the AUTO column tile,
+  // half and twice it, and the unfused N1 passes
+  if (apply_plan(c, PFDD_PLAN_INTERLEAVED) == PFDD_OK) {
+    const int ct = c->c3_tile;
+    int lg = 0;
+    while (ct > 0 && (1 << lg) < ct) lg++;
+    for (int r = 0; r <= 2; r++) {
+      const int rb = (r + 1) << PFDD_PLAN_ROW_TILE_SHIFT;
+      add(PFDD_PLAN_INTERLEAVED | rb | PFDD_PLAN_UNFUSED_COLS);
+      if (ct == 0) continue;
+      for (int d = -1; d <= 1; d++)
+        if (lg + d >= 0) add(PFDD_PLAN_INTERLEAVED | rb | ((lg + d + 1) << PFDD_PLAN_COL_TILE_SHIFT));
+    }
   }
   (void)apply_plan(c, saved);
   return out;
@@ -1418,8 +1599,7 @@ namespace {
 template <typename T>
 int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
   cx<T>* W = (cx<T>*)c->W;
-  const int n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3, F = c->F;
-  const long n = (long)F * c->P;
+  const long n = (long)c->F * c->P;
   int rc = PFDD_OK;
   for (int r = 0; r < reps && rc == PFDD_OK; r++) {
     const int sign = (r & 1) ? PFDD_BACKWARD : PFDD_FORWARD;
@@ -1428,19 +1608,21 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
         rc = c->plane ? plane_launch<T>(c, false) : c->row_tile > 0 ? k1_launch<T>(c, false) : update<T>(c, false);
         break;
       case PFDD_STAGE_FINISH:
-        hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F,
-                           c->plane ? c->p.n1 : c->row_tile > 0 ? c->row_nblk : c->upd_nblk, c->S, c->hist,
-                           (int*)nullptr);
+        if (!c->plane && c->row_tile > 0)
+          finish_rows(c, nullptr);
+        else
+          hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(FIN_THREADS), 0, c->stream, c->partials, c->F,
+                             c->plane ? c->p.n1 : c->upd_nblk, c->S, c->hist, (int*)nullptr);
         rc = hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
         break;
       case PFDD_STAGE_FFT_AXIS3:
-        rc = launch_axis<T>(W, (long)F * n1 * n2, n3, 1, sign, tw_of<T>(c->tw[2]), c->stream, rl_of(c, 2));
+        rc = pass<T>(c, 2, sign);
         break;
       case PFDD_STAGE_FFT_AXIS2:
-        rc = launch_axis<T>(W, (long)F * n1, n2, n3, sign, tw_of<T>(c->tw[1]), c->stream, rl_of(c, 1));
+        rc = pass<T>(c, 1, sign);
         break;
       case PFDD_STAGE_FFT_AXIS1:
-        rc = launch_axis<T>(W, (long)F, n1, (long)n2 * n3, sign, tw_of<T>(c->tw[0]), c->stream, rl_of(c, 0));
+        rc = pass<T>(c, 0, sign);
         break;
       case PFDD_STAGE_CONTRACT:
         rc = c->c3_tile > 0 ? k3_launch<T>(c, 0) : contract<T>(c, 0);
@@ -1472,7 +1654,8 @@ extern "C" int pfdd_step_plan(const pfdd_ctx* c, int* launches) {
   launches[PFDD_STAGE_CONTRACT] = 1;
   launches[PFDD_STAGE_STEP] = 0;
   launches[PFDD_STAGE_COPY] = 0;
-  launches[PFDD_STAGE_FINISH] = c->nline > 0 ? 2 : 1;  // + the line sums while a list is set
+  // + the fold of the interleaved row kernel's block sums, + the line sums while a list is set
+  launches[PFDD_STAGE_FINISH] = 1 + (folds(c) ? 1 : 0) + (c->nline > 0 ? 1 : 0);
   return PFDD_OK;
 }
 

@@ -336,6 +336,23 @@ __device__ __forceinline__ void finish_body(const double* __restrict__ partials,
   }
 }
 
+// First stage of the per-field sums when a field has many block sums (the
+// interleaved row kernel writes one per row block): block (j, f) adds the
+// j-th of FOLD consecutive ranges of field f's block sums in a fixed order,
+// giving FOLD sums per field for k_finish_sums.
+constexpr int FOLD = 16;
+__global__ void __launch_bounds__(256)
+k_fold_partials(const double* __restrict__ in, int nblk, double* __restrict__ out) {
+  __shared__ double red[256 / 64];
+  const int f = blockIdx.y, j = blockIdx.x;
+  const int per = (nblk + FOLD - 1) / FOLD;
+  const int b0 = j * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+  double acc = 0.0;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += 256) acc += in[(long)f * nblk + b];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[(long)f * FOLD + j] = s;
+}
+
 __global__ void __launch_bounds__(FIN_THREADS)
 k_finish_sums(const double* __restrict__ partials, int nfields, int nblk, int ns, double* out, int* step_ctr) {
   __shared__ double fsum[FIN_MAX_FIELDS];
@@ -566,7 +583,8 @@ __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const Elast
 
 // Pointwise contraction in place on W (used when the fused column kernel
 // does not fit).
-template <typename T, int SM, bool STORED>
+// IL: the interleaved layout (fields of a point together).
+template <typename T, int SM, bool STORED, bool IL>
 __global__ void __launch_bounds__(256)
 k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConsts* __restrict__ ec,
            long P, int quirk) {
@@ -577,8 +595,8 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
   cx<T> v[SM];
 #pragma unroll
   for (int b = 0; b < SM; b++)
-    if (b < S) v[b] = W[(long)b * P + pt];
-  contract_point<T, SM, true>(v, ec, pt, S, quirk, STORED, Bsym, P, W + pt, P);
+    if (b < S) v[b] = IL ? W[pt * S + b] : W[(long)b * P + pt];
+  contract_point<T, SM, true>(v, ec, pt, S, quirk, STORED, Bsym, P, IL ? W + pt * S : W + pt, IL ? 1 : P);
 }
 
 // ---------------------------------------------------------------- fused kernels
@@ -625,6 +643,213 @@ k_row_step(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask
   fft_pencils<T, N, 1>(l, l, g, pen, 1 << 30, tile, q, (T)-1, tw);
   const double sum = block_sum(acc, red) * inv_n;
   if (threadIdx.x == 0) partials[(long)f * (rows_per_field / tile) + (pt0 / N) / tile] = sum;
+}
+
+// ---------------------------------------------------------------- interleaved layout
+// W[instance][point][ns]: the fields of a grid point sit together, so the
+// rows (or columns) of every field that one block owns are a single
+// contiguous stretch of W.  The block stages that stretch into LDS pencils
+// with coalesced loads, transforms there and writes it back the same way.
+// One TPP-lane group per pencil; blockDim.x >= pencils * TPP.
+
+// Walks e = (hi * M + mid) * S + lo in steps of `step` with carries instead
+// of divisions (S and M are run-time).
+struct Walk3 {
+  int lo, mid, hi, dlo, dmid, dhi, S, M;
+  __device__ __forceinline__ Walk3(int e, int step, int S_, int M_) : S(S_), M(M_) {
+    lo = e % S;
+    mid = (e / S) % M;
+    hi = e / S / M;
+    dlo = step % S;
+    dmid = (step / S) % M;
+    dhi = step / S / M;
+  }
+  __device__ __forceinline__ void next() {
+    lo += dlo;
+    mid += dmid;
+    hi += dhi;
+    if (lo >= S) {
+      lo -= S;
+      mid++;
+    }
+    if (mid >= M) {
+      mid -= M;
+      hi++;
+    }
+  }
+};
+
+// Staging of a block's stretch of W (chunk = npen * N elements, blockDim.x >=
+// npen * TPP threads, so at most EPT = N / TPP per thread).  Element
+// e = (hi * M + mid) * S + lo sits at g[goff(w)] and at sm[lidx(w)].  With
+// BATCH the thread's elements go in batches (4 of 16 bytes or 8 of 8 bytes)
+// whose loads are all issued before the batch's first store; measured at
+// 128^3 x 12 this helps the FP32 row kernel (0.31 -> 0.22 ms) and slows the
+// FP64 kernels, whose extra registers spill.
+template <typename T, int EPT, bool IN, bool BATCH, typename GOff, typename LIdx>
+__device__ __forceinline__ void il_stage(cx<T>* g, cx<T>* sm, int chunk, int S, int M, GOff goff, LIdx lidx) {
+  if constexpr (!BATCH) {
+    Walk3 w(threadIdx.x, blockDim.x, S, M);
+    for (int e = threadIdx.x; e < chunk; e += blockDim.x, w.next()) {
+      if (IN)
+        sm[lidx(w)] = g[goff(w)];
+      else
+        g[goff(w)] = sm[lidx(w)];
+    }
+  } else {
+    constexpr int BB = (sizeof(T) == 8 ? 4 : 8) < EPT ? (sizeof(T) == 8 ? 4 : 8) : EPT;
+    Walk3 wr(threadIdx.x, blockDim.x, S, M), ww = wr;
+#pragma unroll 1
+    for (int k0 = 0; k0 < EPT; k0 += BB) {
+      cx<T> buf[BB];
+#pragma unroll
+      for (int k = 0; k < BB; k++, wr.next())
+        if (k0 + k < EPT && (int)threadIdx.x + (k0 + k) * (int)blockDim.x < chunk)
+          buf[k] = IN ? g[goff(wr)] : sm[lidx(wr)];
+#pragma unroll
+      for (int k = 0; k < BB; k++, ww.next())
+        if (k0 + k < EPT && (int)threadIdx.x + (k0 + k) * (int)blockDim.x < chunk) {
+          if (IN)
+            sm[lidx(ww)] = buf[k];
+          else
+            g[goff(ww)] = buf[k];
+        }
+    }
+  }
+}
+
+// prologue copy xi -> W in the interleaved layout; n = B * S * P
+template <typename T>
+__global__ void k_xi_to_w_il(Master<T> xi, cx<T>* __restrict__ W, long P, int S, long n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const long ip = e / S;  // instance * P + point
+  const int a = (int)(e - ip * S);
+  const long b = ip / P, pt = ip - b * P;
+  const double2 x = xi.load((b * S + a) * P + pt);
+  W[e] = {(T)x.x, (T)x.y};
+}
+
+// Row kernel, interleaved layout: block (blk, inst) owns rows blk*R ..
+// blk*R+R-1 (along N3) of every field: pencil a*R + r.  Inverse N3
+// transform (G.c:4638), plasticevolv (G.c:3470-3501) with the copy into the
+// FFT array (G.c:4581-4590), forward N3 transform (first pass of G.c:4599).
+// Each field's sum of Re xi / P over the block goes to
+// partials[inst*S + a][blk]; it adds the field's threads in thread order.
+// Dynamic LDS: S*R pencils of pitch N+1, then blockDim.x doubles.
+template <typename T, int N, bool MASK, bool TAUPP>
+__global__ void __launch_bounds__(1024)
+k_row_step_il(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask,
+              const double* __restrict__ taupp, StepArgs a, int R, const cx<T>* __restrict__ tw,
+              double* __restrict__ partials, const int* __restrict__ step_ctr) {
+  constexpr int TPP = make_plan(N).tpp, LD = pitch<N>();
+  extern __shared__ unsigned char lds_raw[];
+  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  const int S = a.ns, npen = S * R, NT = blockDim.x;
+  double* red = reinterpret_cast<double*>(sm + (long)npen * LD);
+  const long P = a.P;
+  const long nblk = P / ((long)N * R);
+  const long blk = blockIdx.x;
+  const int inst = blockIdx.y;
+  const long pt0 = blk * R * N;
+  cx<T>* wb = W + ((long)inst * P + pt0) * S;
+  const int chunk = R * N * S;
+  constexpr int EPT = N / TPP;  // elements per pencil lane
+  constexpr bool BATCH = sizeof(T) == 4;
+  // element (r * N + i) * S + a: pencil a * R + r, position i
+  auto goff = [](const Walk3& w) { return (long)(w.hi * w.M + w.mid) * w.S + w.lo; };
+  auto lidx = [R](const Walk3& w) { return (w.lo * R + w.hi) * LD + w.mid; };
+  il_stage<T, EPT, true, BATCH>(wb, sm, chunk, S, N, goff, lidx);
+  __syncthreads();
+  const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, tw);
+  const double inv_n = 1.0 / (double)P;
+  double acc = 0.0;
+  if (p < npen) {
+    const int fa = p / R, r = p - fa * R, f = inst * S + fa;
+    const FieldCoef c = field_coef(a, f, step_ctr);
+    const long rpt = pt0 + (long)r * N;  // first point of the row
+    const long off = (long)f * P + rpt;
+    constexpr int XB = !BATCH ? 1 : EPT < 4 ? EPT : 4;  // xi loads in flight per batch
+#pragma unroll
+    for (int j0 = 0; j0 < EPT; j0 += XB) {
+      double2 xv[XB];
+#pragma unroll
+      for (int j = 0; j < XB; j++)
+        if (j0 + j < EPT) xv[j] = xi.load(off + q + (j0 + j) * TPP);
+#pragma unroll
+      for (int j = 0; j < XB; j++) {
+        if (j0 + j >= EPT) continue;
+        const int i = q + (j0 + j) * TPP, li = p * LD + i;
+        const cx<T> d = sm[li];
+        double2 x = xv[j];
+        if (!MASK || mask[off + i] == 0) {
+          gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, rpt + i, inv_n);
+          xi.store(off + i, x);
+        }
+        sm[li] = {(T)x.x, (T)x.y};
+        acc += x.x;  // (scaled by 1/P once per field)
+      }
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < S) {
+    const int g = R * TPP;
+    double s = 0.0;
+    for (int k = 0; k < g; k++) s += red[threadIdx.x * g + k];
+    partials[((long)inst * S + threadIdx.x) * nblk + blk] = s * inv_n;
+  }
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)-1, tw);
+  il_stage<T, EPT, false, BATCH>(wb, sm, chunk, S, N, goff, lidx);
+}
+
+template <typename T, int N> constexpr size_t row_il_lds_bytes(int S, int R, int nt) {
+  return (size_t)S * R * pitch<N>() * sizeof(cx<T>) + (size_t)nt * sizeof(double);
+}
+
+// Column kernel, interleaved layout: block (tile bt, inst) owns the `tile`
+// consecutive (i2,i3) columns from bt*tile on, every field: pencil
+// a*tile + p.  Forward N1 transform (last pass of G.c:4599), contraction
+// with the recomputed B(k) (G.c:4609-4629), inverse N1 transform (first
+// pass of G.c:4638).  Dynamic LDS: S*tile pencils of pitch N+1.
+template <typename T, int N, int SM>
+__global__ void __launch_bounds__(1024)
+k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int tile, int quirk,
+                  const cx<T>* __restrict__ tw) {
+  constexpr int TPP = make_plan(N).tpp, LD = pitch<N>();
+  extern __shared__ unsigned char lds_raw[];
+  cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  const int S = ec->ns, npen = S * tile, NT = blockDim.x;
+  const long inner = P / N;
+  const long m0 = xcd_tile(blockIdx.x, gridDim.x) * tile;
+  cx<T>* wb = W + ((long)blockIdx.y * P + m0) * S;  // row n of the tile at wb + n * inner * S
+  const int chunk = N * tile * S;
+  constexpr int EPT = N / TPP;
+  // element (n * tile + c) * S + a: pencil a * tile + c, position n
+  const long rs = inner * S;
+  auto goff = [rs](const Walk3& w) { return (long)w.hi * rs + w.mid * w.S + w.lo; };
+  auto lidx = [tile](const Walk3& w) { return (w.lo * tile + w.mid) * LD + w.hi; };
+  il_stage<T, EPT, true, false>(wb, sm, chunk, S, tile, goff, lidx);
+  __syncthreads();
+  const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)-1, tw);
+  for (int e = threadIdx.x; e < N * tile; e += NT) {
+    const int c = e / N, n = e - c * N;  // consecutive threads read consecutive LDS words
+    cx<T> v[SM];
+#pragma unroll
+    for (int b = 0; b < SM; b++)
+      if (b < S) v[b] = sm[(b * tile + c) * LD + n];
+    contract_point<T, SM, false>(v, ec, (long)n * inner + m0 + c, S, quirk, false, (const T*)nullptr, P,
+                                 sm + c * LD + n, (long)tile * LD);
+  }
+  __syncthreads();
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, tw);
+  il_stage<T, EPT, false, false>(wb, sm, chunk, S, tile, goff, lidx);
+}
+
+template <typename T, int N> constexpr size_t col_il_lds_bytes(int S, int tile) {
+  return (size_t)S * tile * pitch<N>() * sizeof(cx<T>);
 }
 
 // Plane kernel: one block owns the (i2, i3) plane of one field at one i1 and

@@ -164,7 +164,7 @@ def test_plan_tiles_and_errors():
         x, _ = single_run(w, m, 12, pa.FP64, plan=plan)
         assert np.abs(x - base).max() <= 1e-12, plan
     with pa.Context(w.params(0, pa.FP64)) as c:
-        for bad in (pa.plan_row_tile(4096), pa.PLAN_UNFUSED_ROWS | pa.plan_row_tile(2), 1 << 20):
+        for bad in (pa.plan_row_tile(4096), pa.PLAN_UNFUSED_ROWS | pa.plan_row_tile(2), 1 << 24):
             with pytest.raises(pa.PfddError) as ei:
                 c.set_plan(bad)
             assert ei.value.rc == pa.E_SHAPE
