@@ -56,15 +56,16 @@ def rocprof_name(stage, plan, fp64, shape, S, stored):
     if stage == pa.STAGE_UPDATE:
         return (f"void pfdd::k_row_step<{T}, {n3}, false, false>" if plan[pa.STAGE_FFT_AXIS3] == 0
                 else f"void pfdd::k_update<{T}, false, false>")
+    # (the AUTO plan: radix-8 sequences, field-major layout)
     if stage == pa.STAGE_FFT_AXIS3:
-        return f"void pfdd::k_fft_rows<{T}, {n3}>"
+        return f"void pfdd::k_fft_rows<{T}, {n3}, 8>"
     if stage == pa.STAGE_FFT_AXIS2:
-        return f"void pfdd::k_fft_cols<{T}, {n2}>"
+        return f"void pfdd::k_fft_cols<{T}, {n2}, 8>"
     if stage == pa.STAGE_FFT_AXIS1:
-        return f"void pfdd::k_fft_cols<{T}, {n1}>"
+        return f"void pfdd::k_fft_cols<{T}, {n1}, 8>"
     if stage == pa.STAGE_CONTRACT:
         return (f"void pfdd::k_col_contract<{T}, {n1}, {sm}>" if plan[pa.STAGE_FFT_AXIS1] == 0
-                else f"void pfdd::k_contract<{T}, {sm}, {'true' if stored else 'false'}>")
+                else f"void pfdd::k_contract<{T}, {sm}, {'true' if stored else 'false'}, false>")
     return None
 
 

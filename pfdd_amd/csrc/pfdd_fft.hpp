@@ -29,6 +29,30 @@ template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { 
 // (s i) * a for s = +-1
 template <typename T> __device__ __forceinline__ cx<T> mul_si(cx<T> a, T s) { return {-s * a.y, s * a.x}; }
 
+// HBM accesses of the passes that read or write W directly (first and last
+// pass).  Stores are non-temporal, and so are the loads of the contiguous
+// rows (W is streamed: no pass reads what another pass of the same step has
+// left in L2).  Measured at 128^3 x 12: 0.99 vs 1.01 ms per FP64 step,
+// 0.555 vs 0.576 ms mixed; non-temporal loads of the strided columns too
+// were slower (1.02 ms).
+template <typename T> struct VecOf;
+template <> struct VecOf<double> { typedef double t __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<float> { typedef float t __attribute__((ext_vector_type(2))); };
+template <typename T, bool NT> __device__ __forceinline__ cx<T> gload(const cx<T>* p) {
+  if constexpr (NT) {
+    const typename VecOf<T>::t v = __builtin_nontemporal_load(reinterpret_cast<const typename VecOf<T>::t*>(p));
+    return {v.x, v.y};
+  } else {
+    return *p;
+  }
+}
+template <typename T> __device__ __forceinline__ void gstore(cx<T>* p, cx<T> v) {
+  typename VecOf<T>::t w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<typename VecOf<T>::t*>(p));
+}
+
 // ---------------------------------------------------------------- plans
 struct Plan {
   int np;      // number of passes
@@ -146,15 +170,15 @@ template <typename T> struct RowPencils {      // contiguous rows: base[p * n + 
   cx<T>* base;
   int n;
   static constexpr bool in_lds = false;
-  __device__ __forceinline__ cx<T> load(int p, int i) const { return base[(long)p * n + i]; }
-  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { base[(long)p * n + i] = v; }
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return gload<T, true>(base + (long)p * n + i); }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { gstore<T>(base + (long)p * n + i, v); }
 };
 template <typename T> struct ColPencils {      // strided columns: base[i * inner + p]
   cx<T>* base;
   long inner;
   static constexpr bool in_lds = false;
-  __device__ __forceinline__ cx<T> load(int p, int i) const { return base[(long)i * inner + p]; }
-  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { base[(long)i * inner + p] = v; }
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return gload<T, false>(base + (long)i * inner + p); }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { gstore<T>(base + (long)i * inner + p, v); }
 };
 
 template <typename T> struct LdsColPencils {  // columns of a row-major LDS plane: sm[i * ld + p]
