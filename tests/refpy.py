@@ -52,6 +52,8 @@ def lib():
         L.avestrain.restype = cf
         L.ResidualEnergy.argtypes = [vp, vp, ci, ci, ci, cd, cd, cd, cd]
         L.ResidualEnergy.restype = cf
+        L.Energy_calculation.argtypes = [vp, vp, vp, vp, vp, cd, cd, cd, vp, vp, ci, ci, ci]
+        L.Energy_calculation.restype = cf
         _lib = L
     return _lib
 
@@ -293,3 +295,18 @@ def residual_energy(xi, normal, ppoint, D00, D01, D10, D11):
     return float(lib().ResidualEnergy(x.ctypes.data, n.ctypes.data, int(ppoint[0]), int(ppoint[1]),
                                       int(ppoint[2]), D00, D01, D10, D11))
 
+
+
+def energy_calculation(data, eps, C11, C12, C44, normal, ppoint):
+    """The reference Energy_calculation on the FFT array data [NSV][N1][N2][N3]
+    (complex, stored as float like the reference's data)."""
+    d = np.zeros((NSV, N1, N2, N3, 2), dtype=np.float32)
+    d[..., 0] = np.asarray(data).real
+    d[..., 1] = np.asarray(data).imag
+    fx, fy, fz = frec()
+    e = np.ascontiguousarray(eps, dtype=np.float64).reshape(NS, 3, 3)
+    ev = epsv_basis()
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    return float(lib().Energy_calculation(fx.ctypes.data, fy.ctypes.data, fz.ctypes.data, e.ctypes.data,
+                                          ev.ctypes.data, C11, C12, C44, d.ctypes.data, n.ctypes.data,
+                                          int(ppoint[0]), int(ppoint[1]), int(ppoint[2])))

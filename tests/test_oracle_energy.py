@@ -1,6 +1,8 @@
 """The oracle's energy diagnostics on the interface line against the compiled
 reference (avestrain G.c:641-749, ResidualEnergy G.c:3920-3947) at its
-compile-time grid 32x2x32, NS=2, and the line rule against brute force."""
+compile-time grid 32x2x32, NS=2, and the line rule against brute force; and
+that the reference's Energy_calculation (G.c:3026-3467) reads only the
+virtual-strain fields, so it is 0 on the GL path."""
 import itertools
 
 import numpy as np
@@ -68,3 +70,16 @@ def test_residual_energy_vs_reference():
         got = o.residual_energy(pp, 1, *D)
         assert abs(got - ref) <= 1e-6 * max(1.0, abs(ref))
     assert o.residual_energy(pp, 1, 0.0, 0.0, 0.0, 0.0) == 0.0
+
+
+@pytest.mark.skipif(not R.available(), reason="compiled reference oracle/_ref not built")
+def test_energy_calculation_is_zero_without_virtual_strain():
+    rng = np.random.default_rng(11)
+    C11, C12, C44 = 2.0 + 2 * 0.49, 0.98, 1.0
+    eps = rng.standard_normal((R.NS, 3, 3)) * 0.3
+    pp = (R.N1 // 2, 0, R.N3 * 3 // 4)
+    data = np.zeros((R.NSV, R.N1, R.N2, R.N3), dtype=np.complex128)
+    data[:R.NS] = rng.standard_normal((R.NS, R.N1, R.N2, R.N3)) + 1j * rng.standard_normal((R.NS, R.N1, R.N2, R.N3))
+    assert R.energy_calculation(data, eps, C11, C12, C44, NORMAL, pp) == 0.0
+    data[R.NS + 2] = rng.standard_normal((R.N1, R.N2, R.N3)) + 1j * rng.standard_normal((R.N1, R.N2, R.N3))
+    assert R.energy_calculation(data, eps, C11, C12, C44, NORMAL, pp) != 0.0
