@@ -30,16 +30,18 @@ template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { 
 template <typename T> __device__ __forceinline__ cx<T> mul_si(cx<T> a, T s) { return {-s * a.y, s * a.x}; }
 
 // HBM accesses of the passes that read or write W directly (first and last
-// pass).  Stores are non-temporal, and so are the loads of the contiguous
-// rows (W is streamed: no pass reads what another pass of the same step has
-// left in L2).  Measured at 128^3 x 12: 0.99 vs 1.01 ms per FP64 step,
-// 0.555 vs 0.576 ms mixed; non-temporal loads of the strided columns too
-// were slower (1.02 ms).
+// pass).  In FP64 the stores are non-temporal, and so are the loads of the
+// contiguous rows: W (403 MB at 128^3 x 12) is larger than the 256 MB
+// last-level cache and is streamed; measured 0.99 vs 1.01 ms per step
+// (non-temporal loads of the strided columns too: 1.02 ms).  The FP32 W
+// (201 MB there) fits in that cache once the master's accesses bypass it, and
+// ordinary accesses let the next pass find it there: 0.509 vs 0.533 ms per
+// mixed step.
 template <typename T> struct VecOf;
 template <> struct VecOf<double> { typedef double t __attribute__((ext_vector_type(2))); };
 template <> struct VecOf<float> { typedef float t __attribute__((ext_vector_type(2))); };
 template <typename T, bool NT> __device__ __forceinline__ cx<T> gload(const cx<T>* p) {
-  if constexpr (NT) {
+  if constexpr (NT && sizeof(T) == 8) {
     const typename VecOf<T>::t v = __builtin_nontemporal_load(reinterpret_cast<const typename VecOf<T>::t*>(p));
     return {v.x, v.y};
   } else {
@@ -47,10 +49,14 @@ template <typename T, bool NT> __device__ __forceinline__ cx<T> gload(const cx<T
   }
 }
 template <typename T> __device__ __forceinline__ void gstore(cx<T>* p, cx<T> v) {
-  typename VecOf<T>::t w;
-  w.x = v.x;
-  w.y = v.y;
-  __builtin_nontemporal_store(w, reinterpret_cast<typename VecOf<T>::t*>(p));
+  if constexpr (sizeof(T) == 8) {
+    typename VecOf<T>::t w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<typename VecOf<T>::t*>(p));
+  } else {
+    *p = v;
+  }
 }
 
 // ---------------------------------------------------------------- plans
