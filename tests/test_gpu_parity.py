@@ -435,6 +435,28 @@ def test_line_sums_across_history_chunks():
     assert np.all(np.any(ln[1:] != ln[:-1], axis=1))
 
 
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+def test_edge_dipole_stress_vs_volterra_and_oracle(prec):
+    """pfdd_strain_stress on a slipped band (an edge dislocation dipole)
+    against the oracle's fields and the Volterra solution (see
+    tests/test_oracle_dipole.py for the construction and the bounds)."""
+    import test_oracle_dipole as TD
+    say(f"edge dipole stress prec={prec}")
+    shape = (TD.N, 2, TD.N)
+    eps, xi, xd, z0 = TD.dipole_setup(shape)
+    o = O.Oracle(O.make_params(shape, eps, C.C11, C.C12, C.C44, [0.1], [0.0], [1.0]))
+    o.set_xi(xi)
+    _, _, _, sfo = o.strain_stress(np.zeros((3, 3)), fields=True)
+    p = pa.make_params(shape, eps, C.C11, C.C12, C.C44, [0.1], [0.0], [1.0], precision=prec)
+    with pa.Context(p) as c:
+        c.upload_xi(xi)
+        _, avesig, _, sf = c.strain_stress(np.zeros((3, 3)), fields=True)
+    tol = 1e-12 if prec == pa.FP64 else 2e-6
+    assert np.abs(sf - sfo).max() <= tol
+    e_xz, e_xx, e_zz = TD.compare(sf, shape, xd, z0)
+    assert e_xz <= 5e-3 and e_xx <= 0.15 and e_zz <= 0.2
+
+
 def test_error_codes():
     say("error codes")
     w = C.small_fcc((8, 8, 8), ns=2)
