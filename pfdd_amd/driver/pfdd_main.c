@@ -22,7 +22,7 @@
  * 1000 steps), c2 (256^3 x 12, 64 loops, 2000 steps, mixed precision).
  * keys: n1 n2 n3 systems(plane111|fcc) nloops rmin rmax seed sigma_amp tau0
  *       steps nt converge plastic_max precision(fp64|mixed) bmode(recompute|stored)
- *       quirk pfout_every fields(1|0) energy(1|0) outdir
+ *       quirk pfout_every fields(1|0) energy(1|0) plan autotune outdir
  */
 #include <math.h>
 #include <stdio.h>
@@ -54,6 +54,8 @@ typedef struct {
   int converge;         /* 1: stop on plasticconverge, at most plastic_max steps */
   int plastic_max;
   int precision, bmode, quirk;
+  int plan;             /* kernel plan word (PFDD_PLAN_*), 0 = AUTO */
+  int autotune;         /* > 0: pfdd_autotune over that many GL steps before the run */
   int pfout_every;      /* 0: no pfout.dat */
   int fields;           /* 1: strain1D.dat / stress1D.dat per strain call, strain.txt / stress.txt at the end */
   int energy;           /* 1: checkEnergy.dat per outer step, checkEbarrier.dat per GL step */
@@ -124,6 +126,8 @@ static void set_key(config *c, const char *kv)
   else if (!strcmp(key, "precision")) c->precision = !strcmp(v, "mixed") ? PFDD_MIXED : PFDD_FP64;
   else if (!strcmp(key, "bmode")) c->bmode = !strcmp(v, "stored") ? PFDD_B_STORED : PFDD_B_RECOMPUTE;
   else if (!strcmp(key, "quirk")) c->quirk = atoi(v);
+  else if (!strcmp(key, "plan")) c->plan = (int)strtol(v, NULL, 0);
+  else if (!strcmp(key, "autotune")) c->autotune = atoi(v);
   else if (!strcmp(key, "pfout_every")) c->pfout_every = atoi(v);
   else if (!strcmp(key, "fields")) c->fields = atoi(v);
   else if (!strcmp(key, "energy")) c->energy = atoi(v);
@@ -314,6 +318,7 @@ int main(int argc, char **argv)
   p.n1 = cfg.n1; p.n2 = cfg.n2; p.n3 = cfg.n3;
   p.precision = cfg.precision;
   p.bmode = cfg.bmode;
+  p.plan = cfg.plan;
   p.C44 = mu; p.C12 = lam;
   p.C11 = 2. * p.C44 + p.C12; /* MT == 1: isotropic (G.c:4243-4245) */
   p.d1 = p.d2 = p.d3 = 1.0;
@@ -398,6 +403,12 @@ int main(int argc, char **argv)
   die(pfdd_create(&p, &ctx), "pfdd_create");
   die(pfdd_set_tau(ctx, tau, 0), "pfdd_set_tau");
   die(pfdd_upload_xi(ctx, xi, xi_bc, nsv), "pfdd_upload_xi");
+  if (cfg.autotune > 0) {
+    int plan = 0;
+    float ms = 0.f;
+    die(pfdd_autotune(ctx, cfg.autotune, &plan, &ms), "pfdd_autotune");
+    printf("pfdd: autotuned plan 0x%x, %.4f ms per GL step\n", plan, ms);
+  }
 
   FILE *of2 = open_out(&cfg, "stress-strain.dat", "w");
   FILE *ofpfv = open_out(&cfg, "phasefield.dat", "w");

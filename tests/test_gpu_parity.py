@@ -556,6 +556,23 @@ def test_driver_c0_matches_oracle(tmp_path):
     assert np.abs(Bt[:, 2] - np.array(bar)).max() <= 6e-7
 
 
+@pytest.mark.parametrize("args", [["plan=%d" % pa.PLAN_INTERLEAVED], ["autotune=8"]])
+def test_driver_plan_and_autotune_keys(tmp_path, args):
+    """The driver's plan= and autotune= keys: the run still matches the oracle."""
+    import subprocess
+    say(f"driver c0 steps=20 {args}")
+    exe = os.path.join(os.path.dirname(pa.LIB_PATH), "pfdd")
+    r = subprocess.run([exe, "c0", "steps=20", "fields=0", "energy=0", f"outdir={tmp_path}"] + args,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    if args[0].startswith("autotune"):
+        assert "autotuned plan" in r.stdout
+    w = C.c0()
+    xi = np.fromfile(tmp_path / "xi.dat", dtype=np.float64).reshape(w.ns + 9, *w.shape, 2)
+    xo, _, _ = run_oracle(w, 20)
+    assert np.abs(xi[:w.ns, ..., 0] + 1j * xi[:w.ns, ..., 1] - xo).max() <= 1e-12
+
+
 def test_driver_converge_mode_runs():
     import subprocess
     import tempfile

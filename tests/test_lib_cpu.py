@@ -162,6 +162,8 @@ def test_driver_rejects_bad_arguments():
     assert r.returncode == 2 and "unknown preset" in r.stderr
     r = subprocess.run([exe, "c0", "frobnicate=3"], capture_output=True, text=True)
     assert r.returncode == 2 and "unknown key" in r.stderr
+    r = subprocess.run([exe, "c0", "plan=%d" % (1 << 24)], capture_output=True, text=True)
+    assert r.returncode != 0 and "pfdd_create" in r.stderr
 
 
 def test_fft_seam_requires_init():
