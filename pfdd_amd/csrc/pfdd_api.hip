@@ -254,7 +254,7 @@ int update(pfdd_ctx* c, bool finish = true) {
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
-constexpr int kFusedLens[] = {32, 48, 64, 96, 128, 256, 512};
+constexpr int kFusedLens[] = {32, 48, 64, 96, 128, 192, 256, 384, 512};
 constexpr size_t C3_LDS_BUDGET = 104 * 1024;
 // The fused column kernel holds S*tile pencils in LDS; at NS=12 that allows
 // only 4 (FP64) or 8 (FP32) columns and one block per CU, and it measures
@@ -271,7 +271,7 @@ bool fused_len(int n) {
 int row_tile_max(int n, bool fp64) {
   switch (n) {
 #define PFDD_L(NN) case NN: return fp64 ? tile_rows<double, NN>() : tile_rows<float, NN>();
-    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(192) PFDD_L(256) PFDD_L(384) PFDD_L(512)
 #undef PFDD_L
     default: return 0;
   }
@@ -332,7 +332,7 @@ template <typename T>
 int k1il_attr(pfdd_ctx* c) {
   switch (c->p.n3) {
 #define PFDD_L(NN) case NN: return k1il_launch_n<T, NN>(c, true);
-    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(192) PFDD_L(256) PFDD_L(384) PFDD_L(512)
 #undef PFDD_L
     default: return PFDD_E_RADIX;
   }
@@ -371,7 +371,7 @@ int k1_launch(pfdd_ctx* c, bool finish = true) {
     else                              \
       k1_launch_n<T, NN>(c);          \
     break;
-    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(192) PFDD_L(256) PFDD_L(384) PFDD_L(512)
 #undef PFDD_L
     default: return PFDD_E_RADIX;
   }
@@ -430,7 +430,7 @@ int k3_launch(pfdd_ctx* c, int quirk, bool attr_only = false) {
   switch (c->p.n1) {
 #define PFDD_L(NN) \
   case NN: return c->il ? k3il_launch_n<T, NN>(c, quirk, attr_only) : k3_launch_n<T, NN>(c, quirk, attr_only);
-    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(256) PFDD_L(512)
+    PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(192) PFDD_L(256) PFDD_L(384) PFDD_L(512)
 #undef PFDD_L
     default: return PFDD_E_RADIX;
   }

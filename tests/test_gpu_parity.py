@@ -157,6 +157,29 @@ def test_fcc12_small_grids_vs_oracle(shape):
     assert np.abs(gg - go).max() <= 1e-12 * max(1.0, np.abs(go).max())
 
 
+@pytest.mark.parametrize("shape,plan", [((8, 12, 192), pa.PLAN_AUTO), ((192, 8, 12), pa.PLAN_AUTO),
+                                        ((4, 8, 384), pa.PLAN_AUTO), ((384, 4, 8), pa.PLAN_FUSED_COLS),
+                                        ((8, 12, 192), pa.PLAN_INTERLEAVED), ((192, 8, 32), pa.PLAN_INTERLEAVED)])
+def test_fused_kernels_at_192_and_384(shape, plan):
+    """The fused row and column kernels at the lengths 192 = 2^6 3 and 384 = 2^7 3."""
+    say(f"fused kernels {shape} plan={pa.describe_plan(plan)}")
+    w = C.small_fcc(shape, nloops=3, rmin=2.0, rmax=4.0)
+    xo, go, gao = run_oracle(w, 10)
+    with pa.Context(w.params(pa.FP64, pa.B_RECOMPUTE, plan)) as c:
+        c.set_tau(w.tau)
+        c.upload_xi(w.xi)
+        c.begin_outer(True)
+        gg, gag = c.gl_steps(10)
+        sp = c.step_plan()
+        xg = c.download_xi()
+    assert np.abs(xg - xo).max() <= 1e-12
+    assert np.abs(gag - gao).max() <= 1e-12 * max(1.0, np.abs(gao).max())
+    if shape[2] in (192, 384):
+        assert sp[pa.STAGE_FFT_AXIS3] == 0          # the row kernel takes the N3 passes
+    if shape[0] in (192, 384) and plan != pa.PLAN_AUTO:
+        assert sp[pa.STAGE_FFT_AXIS1] == 0          # the fused column kernel takes the N1 passes
+
+
 @pytest.mark.parametrize("quirk", [True, False])
 def test_prologue_quirk_flag(quirk):
     say(f"quirk={quirk}")
