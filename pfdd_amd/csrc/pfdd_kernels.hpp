@@ -357,7 +357,7 @@ __device__ __forceinline__ void finish_body(const double* __restrict__ partials,
 }
 
 // First stage of the per-field sums when a field has many block sums (the
-// interleaved row kernel writes one per row block): block (j, f) adds the
+// row kernels write one per row block): block (j, f) adds the
 // j-th of FOLD consecutive ranges of field f's block sums in a fixed order,
 // giving FOLD sums per field for k_finish_sums.
 constexpr int FOLD = 16;
