@@ -15,6 +15,7 @@
 #include <cstring>
 #include <algorithm>
 #include <functional>
+#include <type_traits>
 #include <new>
 #include <vector>
 
@@ -135,12 +136,22 @@ void launch_axis_n(cx<T>* data, long outer, long inner, int sign, const cx<T>* t
     const long nb = (nrows + TILE - 1) / TILE;
     hipLaunchKernelGGL((k_fft_rows<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, nrows, (T)sign, tw);
   } else {
-    constexpr int TILE = tile_cols<T, N, RL>();
-    int tile = 1;
-    while (tile * 2 <= TILE && inner % (tile * 2) == 0) tile *= 2;
-    const long nb = outer * (inner / tile);
-    hipLaunchKernelGGL((k_fft_cols<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), (cols_lds_bytes<T, N, RL>()), st,
-                       data, inner, tile, (T)sign, tw);
+    // rows at least 1 MB apart (the N1 pass of a large grid) take the wide tile where it differs
+    bool wide = false;
+    if constexpr (has_wide_cols<T, N, RL>()) wide = inner * (long)sizeof(cx<T>) >= (1L << 20);
+    auto go = [&](auto wide_tag) {
+      constexpr bool WIDE = decltype(wide_tag)::value;
+      constexpr int TILE = tile_cols<T, N, RL, WIDE>();
+      int tile = 1;
+      while (tile * 2 <= TILE && inner % (tile * 2) == 0) tile *= 2;
+      const long nb = outer * (inner / tile);
+      hipLaunchKernelGGL((k_fft_cols<T, N, RL, WIDE>), dim3((unsigned)nb), dim3(TILE * TPP),
+                         (cols_lds_bytes<T, N, RL, WIDE>()), st, data, inner, tile, (T)sign, tw);
+    };
+    if (wide)
+      go(std::true_type{});
+    else
+      go(std::false_type{});
   }
 }
 
@@ -163,9 +174,16 @@ template <typename T, int N>
 int cols_attr_n() {
   HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols<T, N, 8>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds_bytes<T, N, 8>()));
-  if constexpr (has_r4_variant(N))
+  if constexpr (has_wide_cols<T, N, 8>())
+    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols<T, N, 8, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds_bytes<T, N, 8, true>()));
+  if constexpr (has_r4_variant(N)) {
     HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols<T, N, 4>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds_bytes<T, N, 4>()));
+    if constexpr (has_wide_cols<T, N, 4>())
+      HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols<T, N, 4, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)cols_lds_bytes<T, N, 4, true>()));
+  }
   return PFDD_OK;
 }
 

@@ -196,19 +196,28 @@ template <typename T, int N, int RL = 8> constexpr int tile_rows() {
 // threads and 80 KB of (dynamic) LDS, so that two blocks still fit on a CU.
 // Measured at 128^3 x 12: FP64 0.956 -> 0.936 ms per step with 512-byte rows;
 // FP32 0.508 ms with 256-byte rows, 0.529 with 512.
-constexpr int COLS_LDS_MAX = 80 * 1024;
-template <typename T, int N, int RL = 8> constexpr int tile_cols() {
+// WIDE tiles, for long lengths along an axis whose rows lie far apart (the
+// N1 pass at N1 >= 384), may take all 160 KB of LDS, one block per CU:
+// 16 FP64 columns at N1 = 512 instead of 8 (c4, 512^3 x 3 FP64: 18.2 ->
+// 16.9 ms per step; the N2 passes of the same length were slower so).
+constexpr int COLS_LDS_MAX = 80 * 1024, COLS_LDS_WIDE = 160 * 1024;
+template <typename T, int N, int RL = 8, bool WIDE = false> constexpr int tile_cols() {
   constexpr int tpp = make_plan(N, RL).tpp;
   constexpr int row_bytes = sizeof(T) == 8 ? 512 : 256;
   int t = pow2_floor(256 / tpp > 0 ? 256 / tpp : 1);
   if (t < row_bytes / (int)sizeof(cx<T>)) t = row_bytes / (int)sizeof(cx<T>);
-  const int lim = COLS_LDS_MAX / (pitch<N>() * (int)sizeof(cx<T>));
+  const int lim = (WIDE ? COLS_LDS_WIDE : COLS_LDS_MAX) / (pitch<N>() * (int)sizeof(cx<T>));
   while (t > lim && t > 1) t /= 2;
   while (t * tpp > 1024 && t > 1) t /= 2;
   return t;
 }
-template <typename T, int N, int RL = 8> constexpr size_t cols_lds_bytes() {
-  return (size_t)tile_cols<T, N, RL>() * pitch<N>() * sizeof(cx<T>);
+template <typename T, int N, int RL = 8, bool WIDE = false> constexpr size_t cols_lds_bytes() {
+  return (size_t)tile_cols<T, N, RL, WIDE>() * pitch<N>() * sizeof(cx<T>);
+}
+// whether a length has a wide tile that differs from the ordinary one (at
+// N = 256 the wide FP64 tile measured no faster: c2, 256^3 x 12)
+template <typename T, int N, int RL> constexpr bool has_wide_cols() {
+  return N >= 384 && tile_cols<T, N, RL, true>() != tile_cols<T, N, RL, false>();
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -240,11 +249,11 @@ k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict
 // takes `tile` consecutive inner indices (tile divides inner, tile <= TILE).
 // Threads run pencil-fastest, so a wave's loads of one row of the tile are
 // contiguous.
-template <typename T, int N, int RL = 8>
-__global__ void __launch_bounds__((tile_cols<T, N, RL>() * make_plan(N, RL).tpp))
+template <typename T, int N, int RL = 8, bool WIDE = false>
+__global__ void __launch_bounds__((tile_cols<T, N, RL, WIDE>() * make_plan(N, RL).tpp))
 k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* __restrict__ tw) {
-  constexpr int TILE = tile_cols<T, N, RL>(), LD = pitch<N>();
-  extern __shared__ unsigned char lds_raw[];  // cols_lds_bytes<T, N, RL>()
+  constexpr int TILE = tile_cols<T, N, RL, WIDE>(), LD = pitch<N>();
+  extern __shared__ unsigned char lds_raw[];  // cols_lds_bytes<T, N, RL, WIDE>()
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   const long tiles_per_outer = inner / tile;
   const long bt = xcd_tile(blockIdx.x, gridDim.x);

@@ -121,6 +121,29 @@ def test_fft_round_trip_256_cubed():
         assert np.abs(y - x).max() < 1e-5
 
 
+def test_fft_wide_n1_tile_vs_numpy():
+    """The N1 pass with the wide FP64 tile (N1 >= 384, rows 1 MB or more
+    apart) against numpy.fft.fftn (same sign, unnormalised), and back."""
+    say("fft 384x256x256 fp64 (wide N1 tile) vs numpy")
+    shape = (384, 256, 256)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    w = C.Workload("fft", shape, ph.fcc_eps([0]), np.zeros((1, 3)), np.zeros((1, 2, 2, 2), complex), np.zeros(1),
+                   np.zeros((3, 3)), 0)
+    with pa.Context(w.params(pa.FP64)) as c:
+        buf = hip.DeviceBuffer.from_array(x)
+        c.fft(buf.ptr, 1, pa.FORWARD, pa.FP64)
+        c.sync()
+        X = buf.to_array(np.complex128, shape)
+        ref = np.fft.fftn(x)
+        assert np.abs(X - ref).max() <= 1e-12 * np.abs(ref).max()
+        del ref
+        c.fft(buf.ptr, 1, pa.BACKWARD, pa.FP64)
+        c.sync()
+        y = buf.to_array(np.complex128, shape) / np.prod(shape)
+        assert np.abs(y - x).max() <= 1e-12
+
+
 # ---------------------------------------------------------------- GL loop (rows a3-a10)
 def test_c0_full_run_matches_golden():
     """c0: 32^3 x 3, 500 GL steps, FP64: complex xi to 1e-10 against the oracle
