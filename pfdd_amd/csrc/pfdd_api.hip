@@ -137,21 +137,22 @@ void launch_axis_n(cx<T>* data, long outer, long inner, int sign, const cx<T>* t
     hipLaunchKernelGGL((k_fft_rows<T, N, RL>), dim3((unsigned)nb), dim3(TILE * TPP), 0, st, data, nrows, (T)sign, tw);
   } else {
     // rows at least 1 MB apart (the N1 pass of a large grid) take the wide tile where it differs
-    bool wide = false;
-    if constexpr (has_wide_cols<T, N, RL>()) wide = inner * (long)sizeof(cx<T>) >= (1L << 20);
     auto go = [&](auto wide_tag) {
       constexpr bool WIDE = decltype(wide_tag)::value;
       constexpr int TILE = tile_cols<T, N, RL, WIDE>();
       int tile = 1;
       while (tile * 2 <= TILE && inner % (tile * 2) == 0) tile *= 2;
       const long nb = outer * (inner / tile);
-      hipLaunchKernelGGL((k_fft_cols<T, N, RL, WIDE>), dim3((unsigned)nb), dim3(TILE * TPP),
+      hipLaunchKernelGGL((k_fft_cols<T, N, RL, WIDE>), dim3((unsigned)nb), dim3(cols_threads<T, N, RL, WIDE>()),
                          (cols_lds_bytes<T, N, RL, WIDE>()), st, data, inner, tile, (T)sign, tw);
     };
-    if (wide)
-      go(std::true_type{});
-    else
-      go(std::false_type{});
+    if constexpr (has_wide_cols<T, N, RL>()) {
+      if (inner * (long)sizeof(cx<T>) >= (1L << 20)) {
+        go(std::true_type{});
+        return;
+      }
+    }
+    go(std::false_type{});
   }
 }
 

@@ -196,10 +196,13 @@ template <typename T, int N, int RL = 8> constexpr int tile_rows() {
 // threads and 80 KB of (dynamic) LDS, so that two blocks still fit on a CU.
 // Measured at 128^3 x 12: FP64 0.956 -> 0.936 ms per step with 512-byte rows;
 // FP32 0.508 ms with 256-byte rows, 0.529 with 512.
-// WIDE tiles, for long lengths along an axis whose rows lie far apart (the
-// N1 pass at N1 >= 384), may take all 160 KB of LDS, one block per CU:
-// 16 FP64 columns at N1 = 512 instead of 8 (c4, 512^3 x 3 FP64: 18.2 ->
-// 16.9 ms per step; the N2 passes of the same length were slower so).
+// WIDE FP64 tiles, for long lengths along an axis whose rows lie far apart
+// (the N1 pass at N1 >= 384), may take all 160 KB of LDS, one block per CU:
+// 16 columns at N1 = 512 instead of 8 (c4, 512^3 x 3 FP64: 18.2 -> 16.9 ms
+// per step; the N2 passes of the same length were slower so).  In FP32 the
+// tile is held to 16 columns there by the 1024 threads of a block; 32
+// columns at two pencils per thread measured slower (4.1 vs 3.6 ms for the
+// two N1 passes of c4).
 constexpr int COLS_LDS_MAX = 80 * 1024, COLS_LDS_WIDE = 160 * 1024;
 template <typename T, int N, int RL = 8, bool WIDE = false> constexpr int tile_cols() {
   constexpr int tpp = make_plan(N, RL).tpp;
@@ -211,13 +214,16 @@ template <typename T, int N, int RL = 8, bool WIDE = false> constexpr int tile_c
   while (t * tpp > 1024 && t > 1) t /= 2;
   return t;
 }
+template <typename T, int N, int RL = 8, bool WIDE = false> constexpr int cols_threads() {
+  return tile_cols<T, N, RL, WIDE>() * make_plan(N, RL).tpp;
+}
 template <typename T, int N, int RL = 8, bool WIDE = false> constexpr size_t cols_lds_bytes() {
   return (size_t)tile_cols<T, N, RL, WIDE>() * pitch<N>() * sizeof(cx<T>);
 }
 // whether a length has a wide tile that differs from the ordinary one (at
 // N = 256 the wide FP64 tile measured no faster: c2, 256^3 x 12)
 template <typename T, int N, int RL> constexpr bool has_wide_cols() {
-  return N >= 384 && tile_cols<T, N, RL, true>() != tile_cols<T, N, RL, false>();
+  return sizeof(T) == 8 && N >= 384 && tile_cols<T, N, RL, true>() != tile_cols<T, N, RL, false>();
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -250,7 +256,7 @@ k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict
 // Threads run pencil-fastest, so a wave's loads of one row of the tile are
 // contiguous.
 template <typename T, int N, int RL = 8, bool WIDE = false>
-__global__ void __launch_bounds__((tile_cols<T, N, RL, WIDE>() * make_plan(N, RL).tpp))
+__global__ void __launch_bounds__((cols_threads<T, N, RL, WIDE>()))
 k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* __restrict__ tw) {
   constexpr int TILE = tile_cols<T, N, RL, WIDE>(), LD = pitch<N>();
   extern __shared__ unsigned char lds_raw[];  // cols_lds_bytes<T, N, RL, WIDE>()
