@@ -230,14 +230,15 @@ int fft3d(pfdd_ctx* c, cx<T>* data, int batch, int sign) {
 
 template <typename T>
 int contract(pfdd_ctx* c, int quirk) {
-  const dim3 grid((unsigned)((c->P + 255) / 256), (unsigned)c->B);
+  constexpr int bs = contract_threads<T>();
+  const dim3 grid((unsigned)((c->P + bs - 1) / bs), (unsigned)c->B);
   cx<T>* W = (cx<T>*)c->W;
   const T* B = (const T*)c->Bsym;
 #define PFDD_CT1(SM, IL)                                                                                       \
   if (c->stored)                                                                                               \
-    hipLaunchKernelGGL((k_contract<T, SM, true, IL>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk); \
+    hipLaunchKernelGGL((k_contract<T, SM, true, IL>), grid, dim3(bs), 0, c->stream, W, B, c->ec, c->P, quirk); \
   else                                                                                                         \
-    hipLaunchKernelGGL((k_contract<T, SM, false, IL>), grid, dim3(256), 0, c->stream, W, B, c->ec, c->P, quirk);
+    hipLaunchKernelGGL((k_contract<T, SM, false, IL>), grid, dim3(bs), 0, c->stream, W, B, c->ec, c->P, quirk);
 #define PFDD_CT(SM)     \
   if (c->il) {          \
     PFDD_CT1(SM, true)  \

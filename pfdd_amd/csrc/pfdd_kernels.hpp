@@ -623,8 +623,11 @@ __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const Elast
 // Pointwise contraction in place on W (used when the fused column kernel
 // does not fit).
 // IL: the interleaved layout (fields of a point together).
+// Threads per block: 128 in FP32 (0.076 -> 0.072 ms at 128^3 x 12), 256 in
+// FP64 (128 measured 0.135 -> 0.143 ms).
+template <typename T> constexpr int contract_threads() { return sizeof(T) == 4 ? 128 : 256; }
 template <typename T, int SM, bool STORED, bool IL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(contract_threads<T>())
 k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConsts* __restrict__ ec,
            long P, int quirk) {
   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
