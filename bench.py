@@ -4,9 +4,9 @@
 
 A step is one GL iteration (update, forward 3D FFT, B(k) contraction, inverse
 3D FFT) over the whole 128^3 x 12 instance, inputs resident in HBM.  Prints one
-JSON line.  value = FP64 GL steps/s; the mixed-precision path (FP32 FFT and
-contraction, compensated master xi updated in FP64) is timed beside it with
-its error against FP64.
+JSON line.  value = FP64 GL steps/s; the mixed-precision paths (FP32 FFT and
+contraction, FP64 update of a compensated FP32 master xi, or of a plain FP32
+one) are timed beside it with their error against FP64.
 """
 import argparse
 import json
@@ -30,10 +30,10 @@ STAGES = [("k_update", pa.STAGE_UPDATE, 1), ("k_fft_rows(N3)", pa.STAGE_FFT_AXIS
           ("k_contract", pa.STAGE_CONTRACT, 1)]
 
 
-def stage_bytes(stage, S, P, ntri, fp64, stored):
+def stage_bytes(stage, S, P, ntri, fp64, stored, lean=False):
     """Algorithmic bytes of one launch: each input read once, each output written once."""
     c = 16 if fp64 else 8
-    m = 16 if fp64 else 12                        # master xi: double2, or the compensated FP32 form
+    m = 16 if fp64 else 8 if lean else 12         # master xi: double2, plain FP32 or the compensated FP32 form
     if stage == pa.STAGE_FINISH:
         return 0
     r = 8 if fp64 else 4
@@ -48,21 +48,22 @@ def stage_bytes(stage, S, P, ntri, fp64, stored):
     raise ValueError(stage)
 
 
-def rocprof_name(stage, plan, fp64, shape, S, stored):
+def rocprof_name(stage, plan, fp64, shape, S, stored, lean=False):
     """The kernel symbol rocprofv3 reports for a stage of this context."""
     T = "double" if fp64 else "float"
+    M = "pfdd::Master<double> " if fp64 else "pfdd::MasterF32L" if lean else "pfdd::MasterF32C"
     n1, n2, n3 = shape
     sm = 4 if S <= 4 else 8 if S <= 8 else 12 if S <= 12 else 16
     if stage == pa.STAGE_UPDATE:
-        return (f"void pfdd::k_row_step<{T}, {n3}, false, false>" if plan[pa.STAGE_FFT_AXIS3] == 0
+        return (f"void pfdd::k_row_step<{T}, {n3}, false, false, {M}>" if plan[pa.STAGE_FFT_AXIS3] == 0
                 else f"void pfdd::k_update<{T}, false, false>")
-    # (the AUTO plan: radix-8 sequences, field-major layout)
+    # (the AUTO plan: radix-8 sequences, field-major layout, narrow column tiles)
     if stage == pa.STAGE_FFT_AXIS3:
         return f"void pfdd::k_fft_rows<{T}, {n3}, 8>"
     if stage == pa.STAGE_FFT_AXIS2:
-        return f"void pfdd::k_fft_cols<{T}, {n2}, 8>"
+        return f"void pfdd::k_fft_cols<{T}, {n2}, 8, false>"
     if stage == pa.STAGE_FFT_AXIS1:
-        return f"void pfdd::k_fft_cols<{T}, {n1}, 8>"
+        return f"void pfdd::k_fft_cols<{T}, {n1}, 8, false>"
     if stage == pa.STAGE_CONTRACT:
         return (f"void pfdd::k_col_contract<{T}, {n1}, {sm}>" if plan[pa.STAGE_FFT_AXIS1] == 0
                 else f"void pfdd::k_contract<{T}, {sm}, {'true' if stored else 'false'}, false>")
@@ -80,12 +81,13 @@ def measured_traffic(name):
     return None if k is None else int(round(k["traffic_bytes"]))
 
 
-def step_alg_bytes(S, P, fp64, stored):
+def step_alg_bytes(S, P, fp64, stored, lean=False):
     """Stage-minimum bytes of one GL step (C2C, FP64: (144 S [+4 S(S+1)]) P;
-    mixed with the 12-byte compensated master: (88 S [+2 S(S+1)]) P)."""
+    mixed with the 12-byte compensated master: (88 S [+2 S(S+1)]) P; with the
+    8-byte plain FP32 master: (80 S [+2 S(S+1)]) P)."""
     if fp64:
         return (144 * S + (4 * S * (S + 1) if stored else 0)) * P
-    return (88 * S + (2 * S * (S + 1) if stored else 0)) * P
+    return ((80 if lean else 88) * S + (2 * S * (S + 1) if stored else 0)) * P
 
 
 def timed_steps(ctx, steps, warmup):
@@ -229,9 +231,11 @@ def main():
            "gridpoints_x_systems_per_s": round(P * S * sps64, 1),
            "kernels_fp64": kernels, "wall_s_fp64": round(wall64, 3)}
 
-    # ---- mixed precision
-    if not a.no_mixed:
-        ctxm = make_ctx(w, pa.MIXED, bmode)
+    # ---- mixed precision: compensated FP32 master ("mixed"), plain FP32 master ("mixed_lean")
+    for key, prec, lean in (("mixed", pa.MIXED, False), ("mixed_lean", pa.MIXED_LEAN, True)):
+        if a.no_mixed:
+            break
+        ctxm = make_ctx(w, prec, bmode)
         ms32, _ = timed_steps(ctxm, a.steps, a.warmup)
         xi32 = ctxm.download_xi()
         stages32 = time_kernels(ctxm)
@@ -240,15 +244,17 @@ def main():
         err_xi = float(np.abs(xi32 - xi64).max())
         area64 = float(xi64.real.sum())
         area32 = float(xi32.real.sum())
-        out["mixed"] = {"value": round(sps32, 2), "unit": "GL steps/s", "speedup_vs_fp64": round(sps32 / sps64, 3),
-                        "xi_max_abs_err_vs_fp64": err_xi,
-                        "slipped_area_rel_err_vs_fp64": abs(area32 - area64) / abs(area64),
-                        "after_steps": a.steps + a.warmup,
-                        "step_roofline_frac": round(step_alg_bytes(S, P, False, stored) * sps32 / 1e9 / HBM_PEAK_GBS, 4),
-                        "kernels_mixed": {n: {"avg_ms": round(ms, 5), "launches_per_step": k,
-                                              "GB_s": round(stage_bytes(st, S, P, ntri, False, stored)
-                                                            / (ms / 1e3) / 1e9, 1)}
-                                          for n, (st, k, ms) in stages32.items()}}
+        out[key] = {"value": round(sps32, 2), "unit": "GL steps/s", "speedup_vs_fp64": round(sps32 / sps64, 3),
+                    "master_xi": "plain FP32 complex, 8 B" if lean else "compensated FP32 (hi + lo, im), 12 B",
+                    "xi_max_abs_err_vs_fp64": err_xi,
+                    "slipped_area_rel_err_vs_fp64": abs(area32 - area64) / abs(area64),
+                    "after_steps": a.steps + a.warmup,
+                    "step_roofline_frac": round(step_alg_bytes(S, P, False, stored, lean) * sps32 / 1e9
+                                                / HBM_PEAK_GBS, 4),
+                    "kernels": {n: {"avg_ms": round(ms, 5), "launches_per_step": k,
+                                    "GB_s": round(stage_bytes(st, S, P, ntri, False, stored, lean)
+                                                  / (ms / 1e3) / 1e9, 1)}
+                                for n, (st, k, ms) in stages32.items()}}
         del xi32
     del xi64
 

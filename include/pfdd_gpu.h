@@ -39,7 +39,12 @@ enum {
   PFDD_E_STATE = -7         /* call order (e.g. steps before upload) */
 };
 
-enum { PFDD_FP64 = 0, PFDD_MIXED = 1 };          /* precision of FFT + contraction */
+/* Precision of FFT + contraction.  PFDD_MIXED: FP32 FFTs and contraction,
+ * FP64 update on a compensated FP32 master xi (12 B a point, Re xi to ~48
+ * bits).  PFDD_MIXED_LEAN: the same with xi kept as a plain FP32 complex
+ * (8 B a point), rounded to FP32 after every update, and the update's sine
+ * taken in FP32 after an FP64 reduction of its argument mod 2 pi. */
+enum { PFDD_FP64 = 0, PFDD_MIXED = 1, PFDD_MIXED_LEAN = 2 };
 enum { PFDD_B_STORED = 0, PFDD_B_RECOMPUTE = 1 }; /* interaction matrix B(k) */
 enum { PFDD_FORWARD = -1, PFDD_BACKWARD = 1 };    /* FFT sign (G.c:106, FFTW_FORWARD) */
 
@@ -48,7 +53,7 @@ enum { PFDD_FORWARD = -1, PFDD_BACKWARD = 1 };    /* FFT sign (G.c:106, FFTW_FOR
 typedef struct {
   int n1, n2, n3;              /* grid; each edge 2^a 3^b, 1 < n <= 1024 */
   int ns;                      /* active slip systems, 1..PFDD_MAX_NS */
-  int precision;               /* PFDD_FP64 | PFDD_MIXED */
+  int precision;               /* PFDD_FP64 | PFDD_MIXED | PFDD_MIXED_LEAN */
   int bmode;                   /* PFDD_B_STORED | PFDD_B_RECOMPUTE */
   double C11, C12, C44;        /* elastic constants; mu = C44 (G.c:337) */
   double d1, d2, d3;           /* grid spacing of frec (G.c:4266-4268) */
@@ -167,7 +172,8 @@ int pfdd_strain_stress(pfdd_ctx *c, const double *avepst_uniform, double *aveps,
 
 /* Raw FFT seam (fft_forward / fft_backward, G.c:209-273): batched, in place,
  * unnormalised 3D C2C on a device buffer of batch contiguous N1*N2*N3 complex
- * blocks.  dev_buf is double2 when prec == PFDD_FP64, float2 otherwise.
+ * blocks.  dev_buf is double2 when prec == PFDD_FP64, float2 when
+ * prec == PFDD_MIXED (PFDD_E_SHAPE for other values).
  * dir = PFDD_FORWARD (-1) or PFDD_BACKWARD (+1).  Asynchronous. */
 int pfdd_fft(pfdd_ctx *c, void *dev_buf, int batch, int dir, int prec);
 

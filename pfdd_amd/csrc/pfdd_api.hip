@@ -51,6 +51,7 @@ struct pfdd_ctx {
   int F = 0;                 // fields, B * S
   int ntri = 0;
   int fp64 = 1;
+  int lean = 0;              // PFDD_MIXED_LEAN: plain FP32 master
   int stored = 0;
   hipStream_t stream = nullptr;
   int owns_stream = 1;       // 0: the stream belongs to an ensemble
@@ -122,7 +123,19 @@ template <typename T> const cx<T>* tw_of(const Twiddles& t);
 template <> const double2* tw_of<double>(const Twiddles& t) { return t.d; }
 template <> const float2* tw_of<float>(const Twiddles& t) { return t.f; }
 
-template <typename T> Master<T> xm(const pfdd_ctx* c) { return make_master<T>(c->xi, (long)c->F * c->P); }
+template <typename T> Master<T> xm(const pfdd_ctx* c) { return make_master<T>(c->xi, (long)c->F * c->P, c->lean != 0); }
+// Calls f with the context's master as the static form the per-step kernels
+// are instantiated for (Master<double>, MasterF32C or MasterF32L).
+template <typename T, typename Fn> void with_master(const pfdd_ctx* c, Fn&& f) {
+  const Master<T> m = xm<T>(c);
+  if constexpr (sizeof(T) == 8) {
+    f(m);
+  } else if (m.im) {
+    f(MasterF32C{m.r, m.im});
+  } else {
+    f(MasterF32L{m.r});
+  }
+}
 
 // radix limit of the standalone passes along axis ax (0: N1, 1: N2, 2: N3)
 int rl_of(const pfdd_ctx* c, int ax) { return (c->r4 >> ax) & 1 ? 4 : 8; }
@@ -303,15 +316,18 @@ void k1_launch_n(pfdd_ctx* c) {
   const long nb = (long)c->F * c->row_nblk;
   constexpr int NT = tile_rows<T, N>() * make_plan(N).tpp;
   const cx<T>* tw = tw_of<T>(c->tw[2]);
-#define PFDD_K1(MM, TT)                                                                                   \
-  hipLaunchKernelGGL((k_row_step<T, N, MM, TT>), dim3((unsigned)nb), dim3(NT), 0, c->stream, W, xm<T>(c), c->mask, \
+  with_master<T>(c, [&](auto xi) {
+    using M = decltype(xi);
+#define PFDD_K1(MM, TT)                                                                                          \
+  hipLaunchKernelGGL((k_row_step<T, N, MM, TT, M>), dim3((unsigned)nb), dim3(NT), 0, c->stream, W, xi, c->mask, \
                      c->taupp, c->sa, c->row_tile, tw, c->partials, c->step_ctr)
-  const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
-  if (m && tp) PFDD_K1(true, true);
-  else if (m) PFDD_K1(true, false);
-  else if (tp) PFDD_K1(false, true);
-  else PFDD_K1(false, false);
+    const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+    if (m && tp) PFDD_K1(true, true);
+    else if (m) PFDD_K1(true, false);
+    else if (tp) PFDD_K1(false, true);
+    else PFDD_K1(false, false);
 #undef PFDD_K1
+  });
 }
 
 // Threads of an interleaved-layout block: one TPP-lane group per pencil, in
@@ -326,26 +342,30 @@ int k1il_launch_n(pfdd_ctx* c, bool attr_only) {
   const size_t lds = row_il_lds_bytes<T, N>(c->S, c->row_tile, nt);
   cx<T>* W = (cx<T>*)c->W;
   const cx<T>* tw = tw_of<T>(c->tw[2]);
-#define PFDD_K1(MM, TT)                                                                                           \
-  if (attr_only) {                                                                                                \
-    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step_il<T, N, MM, TT>),                        \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                            \
-  } else {                                                                                                        \
-    hipLaunchKernelGGL((k_row_step_il<T, N, MM, TT>), dim3((unsigned)c->row_nblk, (unsigned)c->B), dim3(nt), lds, \
-                       c->stream, W, xm<T>(c), c->mask, c->taupp, c->sa, c->row_tile, tw, c->partials,            \
-                       c->step_ctr);                                                                              \
+  int rc = PFDD_OK;
+  with_master<T>(c, [&](auto xi) {
+    using M = decltype(xi);
+#define PFDD_K1(MM, TT)                                                                                              \
+  if (attr_only) {                                                                                                   \
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_row_step_il<T, N, MM, TT, M>),                          \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)                    \
+      rc = PFDD_E_HIP;                                                                                               \
+  } else {                                                                                                           \
+    hipLaunchKernelGGL((k_row_step_il<T, N, MM, TT, M>), dim3((unsigned)c->row_nblk, (unsigned)c->B), dim3(nt), lds, \
+                       c->stream, W, xi, c->mask, c->taupp, c->sa, c->row_tile, tw, c->partials, c->step_ctr);     \
   }
-  if (attr_only) {
-    PFDD_K1(true, true) PFDD_K1(true, false) PFDD_K1(false, true) PFDD_K1(false, false)
-  } else {
-    const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
-    if (m && tp) { PFDD_K1(true, true) }
-    else if (m) { PFDD_K1(true, false) }
-    else if (tp) { PFDD_K1(false, true) }
-    else { PFDD_K1(false, false) }
-  }
+    if (attr_only) {
+      PFDD_K1(true, true) PFDD_K1(true, false) PFDD_K1(false, true) PFDD_K1(false, false)
+    } else {
+      const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+      if (m && tp) { PFDD_K1(true, true) }
+      else if (m) { PFDD_K1(true, false) }
+      else if (tp) { PFDD_K1(false, true) }
+      else { PFDD_K1(false, false) }
+    }
 #undef PFDD_K1
-  return PFDD_OK;
+  });
+  return rc;
 }
 
 template <typename T>
@@ -470,24 +490,30 @@ int plane_launch_n(pfdd_ctx* c, bool attr_only) {
   const cx<T>* tw3 = tw_of<T>(c->tw[2]);
   const unsigned nb = (unsigned)(c->F * c->p.n1);
   cx<T>* W = (cx<T>*)c->W;
-#define PFDD_PL(MM, TT)                                                                                          \
-  if (attr_only) {                                                                                               \
-    HIPCK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_step<T, N, N, MM, TT>),                     \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                           \
-  } else {                                                                                                       \
-    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT>), dim3(nb), dim3(plane_threads<T, N, N>()), lds, c->stream, W, xm<T>(c),  \
-                       c->mask, c->taupp, c->sa, tw2, tw3, c->partials, c->step_ctr);                           \
+  int rc = PFDD_OK;
+  with_master<T>(c, [&](auto xi) {
+    using M = decltype(xi);
+#define PFDD_PL(MM, TT)                                                                                             \
+  if (attr_only) {                                                                                                  \
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_step<T, N, N, MM, TT, M>),                       \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)                   \
+      rc = PFDD_E_HIP;                                                                                              \
+  } else {                                                                                                          \
+    hipLaunchKernelGGL((k_plane_step<T, N, N, MM, TT, M>), dim3(nb), dim3(plane_threads<T, N, N>()), lds, c->stream, \
+                       W, xi, c->mask, c->taupp, c->sa, tw2, tw3, c->partials, c->step_ctr);                       \
   }
-  if (attr_only) {
-    PFDD_PL(true, true) PFDD_PL(true, false) PFDD_PL(false, true) PFDD_PL(false, false)
-  } else {
-    const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
-    if (m && tp) { PFDD_PL(true, true) }
-    else if (m) { PFDD_PL(true, false) }
-    else if (tp) { PFDD_PL(false, true) }
-    else { PFDD_PL(false, false) }
-  }
+    if (attr_only) {
+      PFDD_PL(true, true) PFDD_PL(true, false) PFDD_PL(false, true) PFDD_PL(false, false)
+    } else {
+      const bool m = c->mask != nullptr, tp = c->taupp != nullptr;
+      if (m && tp) { PFDD_PL(true, true) }
+      else if (m) { PFDD_PL(true, false) }
+      else if (tp) { PFDD_PL(false, true) }
+      else { PFDD_PL(false, false) }
+    }
 #undef PFDD_PL
+  });
+  if (rc) return rc;
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
@@ -916,7 +942,8 @@ int apply_plan(pfdd_ctx* c, int plan) {
 
 int validate_params(const pfdd_params* p) {
   if (p->ns < 1 || p->ns > PFDD_MAX_NS) return PFDD_E_SHAPE;
-  if (p->precision != PFDD_FP64 && p->precision != PFDD_MIXED) return PFDD_E_SHAPE;
+  if (p->precision != PFDD_FP64 && p->precision != PFDD_MIXED && p->precision != PFDD_MIXED_LEAN)
+    return PFDD_E_SHAPE;
   if (p->bmode != PFDD_B_STORED && p->bmode != PFDD_B_RECOMPUTE) return PFDD_E_SHAPE;
   const int ns3[3] = {p->n1, p->n2, p->n3};
   for (int i = 0; i < 3; i++)
@@ -957,6 +984,7 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
   c->P = (long)p->n1 * p->n2 * p->n3;
   c->ntri = c->S * (c->S + 1) / 2;
   c->fp64 = p->precision == PFDD_FP64;
+  c->lean = p->precision == PFDD_MIXED_LEAN;
   c->stored = p->bmode == PFDD_B_STORED;
   c->tauh.assign((size_t)c->F, make_double2(0.0, 0.0));
   c->inst_pp.assign((size_t)batch, 0);
@@ -982,7 +1010,7 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     c->partials_len = (size_t)std::max<long>(std::max<long>((long)F * c->upd_nblk, 18 * strain_nblk),
                                              std::max<long>((long)F * p->n1 * p->n2, (long)F * p->n1)) +
                       (size_t)F * FOLD;
-    c->xi_bytes = (c->fp64 ? master_bytes<double>() : master_bytes<float>()) * F * P;
+    c->xi_bytes = (c->fp64 ? master_bytes<double>(false) : master_bytes<float>(c->lean != 0)) * F * P;
     if (!(c->xi = dev_alloc<char>(c->xi_bytes, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->W = dev_alloc<char>(csz * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -1163,18 +1191,22 @@ int upload_inst(pfdd_ctx* c, int b, const double* xi, const double* xi_bc, int n
   HIPCK(hipStreamSynchronize(c->stream));
   if (c->fp64) {
     HIPCK(hipMemcpyAsync((double2*)c->xi + off, xi, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
-  } else {  // split into the compensated form of Master<float>
+  } else {  // split into the compensated form of Master<float> (or round, in the lean form)
     std::vector<float2> r(n);
-    std::vector<float> im(n);
+    std::vector<float> im(c->lean ? 0 : n);
     for (size_t i = 0; i < n; i++) {
       const double re = xi[2 * i];
       const float hi = (float)re;
-      r[i] = make_float2(hi, (float)(re - (double)hi));
-      im[i] = (float)xi[2 * i + 1];
+      if (c->lean) {
+        r[i] = make_float2(hi, (float)xi[2 * i + 1]);
+      } else {
+        r[i] = make_float2(hi, (float)(re - (double)hi));
+        im[i] = (float)xi[2 * i + 1];
+      }
     }
     const Master<float> m = xm<float>(c);
     HIPCK(hipMemcpyAsync(m.r + off, r.data(), sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCK(hipMemcpyAsync(m.im + off, im.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    if (m.im) HIPCK(hipMemcpyAsync(m.im + off, im.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
   }
   c->inst_frozen[b] = frozen ? 1 : 0;
@@ -1216,14 +1248,20 @@ int download_inst(pfdd_ctx* c, int b, double* xi, int nsv) {
     HIPCK(hipStreamSynchronize(c->stream));
   } else {
     std::vector<float2> r(n);
-    std::vector<float> im(n);
     const Master<float> m = xm<float>(c);
+    std::vector<float> im(m.im ? n : 0);
     HIPCK(hipMemcpyAsync(r.data(), m.r + (size_t)b * n, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
-    HIPCK(hipMemcpyAsync(im.data(), m.im + (size_t)b * n, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    if (m.im)
+      HIPCK(hipMemcpyAsync(im.data(), m.im + (size_t)b * n, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < n; i++) {
-      xi[2 * i] = (double)r[i].x + (double)r[i].y;
-      xi[2 * i + 1] = (double)im[i];
+      if (m.im) {
+        xi[2 * i] = (double)r[i].x + (double)r[i].y;
+        xi[2 * i + 1] = (double)im[i];
+      } else {
+        xi[2 * i] = (double)r[i].x;
+        xi[2 * i + 1] = (double)r[i].y;
+      }
     }
   }
   if (nsv > c->S) memset(xi + 2 * n, 0, sizeof(double) * 2 * (size_t)(nsv - c->S) * P);
@@ -1837,6 +1875,7 @@ int pers_build(pfdd_ens* e) {
       EnsClassDev& d = h.cls[k];
       d.W = c->W;
       d.xi = c->xi;
+      d.lean = c->lean;
       d.ec = c->ec;
       for (int ax = 0; ax < 3; ax++) d.tw[ax] = fp ? (const void*)c->tw[ax].d : (const void*)c->tw[ax].f;
       d.sa = c->sa;

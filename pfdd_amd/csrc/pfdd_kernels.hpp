@@ -51,8 +51,11 @@ template <> struct Master<double> {
   }
   __device__ __forceinline__ double re(long i) const { return p[i].x; }
 };
-template <> struct Master<float> {
-  float2* r;  // (hi, lo) of Re xi
+// The two FP32 forms as separate types, for the per-step kernels, which are
+// instantiated per form (a per-access branch on the form measured 0.155 ->
+// 0.183 ms for the lean FP32 row kernel at 128^3 x 12).
+struct MasterF32C {  // compensated: (hi, lo) of Re xi, and Im xi
+  float2* r;
   float* im;
   __device__ __forceinline__ double2 load(long i) const {
     const VecOf<float>::t h = xld(reinterpret_cast<const VecOf<float>::t*>(r + i));
@@ -71,15 +74,46 @@ template <> struct Master<float> {
     return (double)h.x + (double)h.y;
   }
 };
-// Bytes per point of the master, and the master over a base allocation of n points.
-template <typename T> constexpr size_t master_bytes() { return sizeof(T) == 8 ? 16 : 12; }
-template <typename T> __host__ __device__ inline Master<T> make_master(void* base, long n);
-template <> __host__ __device__ inline Master<double> make_master<double>(void* base, long) {
+struct MasterF32L {  // lean: (Re xi, Im xi) as a plain FP32 complex
+  float2* r;
+  __device__ __forceinline__ double2 load(long i) const {
+    const VecOf<float>::t h = xld(reinterpret_cast<const VecOf<float>::t*>(r + i));
+    return make_double2((double)h.x, (double)h.y);
+  }
+  __device__ __forceinline__ void store(long i, double2 v) const {
+    VecOf<float>::t w;
+    w.x = (float)v.x;
+    w.y = (float)v.y;
+    xst(reinterpret_cast<VecOf<float>::t*>(r + i), w);
+  }
+  __device__ __forceinline__ double re(long i) const { return (double)r[i].x; }
+};
+// Either form, chosen per access (the kernels that run once per outer
+// iteration or only in non-default plans).
+template <> struct Master<float> {
+  float2* r;  // (hi, lo) of Re xi; (Re xi, Im xi) in the lean form
+  float* im;  // Im xi; null in the lean form
+  __device__ __forceinline__ double2 load(long i) const {
+    return im ? MasterF32C{r, im}.load(i) : MasterF32L{r}.load(i);
+  }
+  __device__ __forceinline__ void store(long i, double2 v) const {
+    if (im) MasterF32C{r, im}.store(i, v);
+    else MasterF32L{r}.store(i, v);
+  }
+  __device__ __forceinline__ double re(long i) const { return im ? MasterF32C{r, im}.re(i) : MasterF32L{r}.re(i); }
+};
+// Bytes per point of the master, and the master over a base allocation of n
+// points.  The lean form (PFDD_MIXED_LEAN) keeps xi as a plain FP32 complex,
+// 8 bytes a point: xi is rounded to FP32 after every update (the update
+// itself still runs in FP64).
+template <typename T> constexpr size_t master_bytes(bool lean) { return sizeof(T) == 8 ? 16 : lean ? 8 : 12; }
+template <typename T> __host__ __device__ inline Master<T> make_master(void* base, long n, bool lean);
+template <> __host__ __device__ inline Master<double> make_master<double>(void* base, long, bool) {
   return Master<double>{reinterpret_cast<double2*>(base)};
 }
-template <> __host__ __device__ inline Master<float> make_master<float>(void* base, long n) {
+template <> __host__ __device__ inline Master<float> make_master<float>(void* base, long n, bool lean) {
   float2* r = reinterpret_cast<float2*>(base);
-  return Master<float>{r, reinterpret_cast<float*>(r + n)};
+  return Master<float>{r, lean ? nullptr : reinterpret_cast<float*>(r + n)};
 }
 
 // Constants of the elastic kernel, shared by every k-point (read through the
@@ -301,15 +335,30 @@ __device__ __forceinline__ FieldCoef field_coef(const StepArgs& a, int f, const 
 }
 // The update of one point (G.c:3485-3489): d is the unnormalised inverse
 // transform of data2.
-template <bool TAUPP>
 // inv_n = 1 / P (exact for the power-of-two grids; within an ulp of the
-// reference's division by nsize otherwise).
+// reference's division by nsize otherwise).  F32SIN (the lean FP32 master
+// only, whose xi carries 24 bits): the sine is taken in FP32 after reducing
+// its FP64 argument mod 2 pi in FP64 (Cody-Waite, two-part 2 pi), so the
+// argument keeps the reference's pi constant and the error stays that of one
+// FP32 sine (measured at 128^3 x 12: 171 -> 157 us for the lean row kernel).
+template <bool TAUPP, bool F32SIN = false>
 __device__ __forceinline__ void gl_point(double2& x, double dre, double dim, const FieldCoef& c, const StepArgs& a,
                                          const double* __restrict__ taupp, long pt, double inv_n) {
   const double t = TAUPP ? taupp[c.tbase + pt * a.ns] + c.rate : c.tau;
-  x.x -= c.cd2 * (a.uq2 * dre * inv_n + c.asf2 * a.pi * sin(2.0 * a.pi * x.x) - t / c.dsl);
+  const double arg = 2.0 * a.pi * x.x;
+  double sn;
+  if constexpr (F32SIN) {
+    constexpr double TWO_PI_HI = 6.283185307179586, TWO_PI_LO = 2.4492935982947064e-16;
+    const double k = rint(arg * (1.0 / TWO_PI_HI));
+    const double r = fma(-k, TWO_PI_LO, fma(-k, TWO_PI_HI, arg));
+    sn = (double)sinf((float)r);
+  } else {
+    sn = sin(arg);
+  }
+  x.x -= c.cd2 * (a.uq2 * dre * inv_n + c.asf2 * a.pi * sn - t / c.dsl);
   x.y -= a.uq2 * c.cd2 * (dim * inv_n);
 }
+template <typename M> constexpr bool lean_master() { return __is_same(M, MasterF32L); }
 
 constexpr int UPD_THREADS = 256, UPD_EPT = 4;
 
@@ -416,7 +465,12 @@ k_update(Master<T> xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
       double2 x = xi.load(off + pt);
       const cx<T> d = W[off + pt];
       if (!MASK || mask[off + pt] == 0) {
-        gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
+        if constexpr (sizeof(T) == 4) {
+          if (!xi.im) gl_point<TAUPP, true>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
+          else gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
+        } else {
+          gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
+        }
         xi.store(off + pt, x);
       }
       W[off + pt] = {(T)x.x, (T)x.y};
@@ -651,9 +705,9 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
 // (G.c:3470-3501) updates xi, the new xi is copied into the rows
 // (G.c:4581-4590) and transformed forward along N3 (first pass of
 // G.c:4599).  The block's sum of Re xi / P goes to partials[f][block].
-template <typename T, int N, bool MASK, bool TAUPP>
+template <typename T, int N, bool MASK, bool TAUPP, typename M = Master<T>>
 __global__ void __launch_bounds__((tile_rows<T, N>() * make_plan(N).tpp))
-k_row_step(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask,
+k_row_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
            const double* __restrict__ taupp, StepArgs a, int tile, const cx<T>* __restrict__ tw,
            double* __restrict__ partials, const int* __restrict__ step_ctr) {
   constexpr int TPP = make_plan(N).tpp, TILE = tile_rows<T, N>(), NT = TILE * TPP, LD = pitch<N>();
@@ -696,7 +750,7 @@ k_row_step(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask
     const cx<T> d = sm[li];
     double2 x = EARLY ? xv[EARLY ? k : 0] : xi.load(off + e);
     if (!MASK || mask[off + e] == 0) {
-      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt0 + e, inv_n);
+      gl_point<TAUPP, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, taupp, pt0 + e, inv_n);
       xi.store(off + e, x);
     }
     sm[li] = {(T)x.x, (T)x.y};
@@ -800,9 +854,9 @@ __global__ void k_xi_to_w_il(Master<T> xi, cx<T>* __restrict__ W, long P, int S,
 // Each field's sum of Re xi / P over the block goes to
 // partials[inst*S + a][blk]; it adds the field's threads in thread order.
 // Dynamic LDS: S*R pencils of pitch N+1, then blockDim.x doubles.
-template <typename T, int N, bool MASK, bool TAUPP>
+template <typename T, int N, bool MASK, bool TAUPP, typename M = Master<T>>
 __global__ void __launch_bounds__(1024)
-k_row_step_il(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask,
+k_row_step_il(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
               const double* __restrict__ taupp, StepArgs a, int R, const cx<T>* __restrict__ tw,
               double* __restrict__ partials, const int* __restrict__ step_ctr) {
   constexpr int TPP = make_plan(N).tpp, LD = pitch<N>();
@@ -847,7 +901,7 @@ k_row_step_il(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ m
         const cx<T> d = sm[li];
         double2 x = xv[j];
         if (!MASK || mask[off + i] == 0) {
-          gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, rpt + i, inv_n);
+          gl_point<TAUPP, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, taupp, rpt + i, inv_n);
           xi.store(off + i, x);
         }
         sm[li] = {(T)x.x, (T)x.y};
@@ -930,8 +984,8 @@ template <typename T, int N2, int N3>
 constexpr int plane_threads() { return (sizeof(T) == 4 && N2 * N3 >= 16384) ? 512 : 1024; }
 
 // The block's work for plane `blk` (= field * N1 + i1); sm is the LDS plane.
-template <typename T, int N2, int N3, bool MASK, bool TAUPP>
-__device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, Master<T> xi,
+template <typename T, int N2, int N3, bool MASK, bool TAUPP, typename M>
+__device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, M xi,
                                            const uint8_t* __restrict__ mask, const double* __restrict__ taupp,
                                            const StepArgs& a, const cx<T>* __restrict__ tw2,
                                            const cx<T>* __restrict__ tw3, double* __restrict__ partials,
@@ -982,7 +1036,7 @@ __device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, Master<T> xi,
     const int li = (e / N3) * LD + (e % N3);
     const cx<T> d = sm[li];
     if (!MASK || mask[off + e] == 0) {
-      gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, (long)i1 * PLANE + e, inv_n);
+      gl_point<TAUPP, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, taupp, (long)i1 * PLANE + e, inv_n);
       xi.store(off + e, x);
     }
     sm[li] = {(T)x.x, (T)x.y};
@@ -1004,13 +1058,13 @@ __device__ __forceinline__ void plane_body(cx<T>* __restrict__ W, Master<T> xi,
   if (t == 0) partials[(long)f * n1 + i1] = sum;
 }
 
-template <typename T, int N2, int N3, bool MASK, bool TAUPP>
+template <typename T, int N2, int N3, bool MASK, bool TAUPP, typename M = Master<T>>
 __global__ void __launch_bounds__((plane_threads<T, N2, N3>()))
-k_plane_step(cx<T>* __restrict__ W, Master<T> xi, const uint8_t* __restrict__ mask,
+k_plane_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
              const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
              const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
   extern __shared__ unsigned char lds_raw[];
-  plane_body<T, N2, N3, MASK, TAUPP>(W, xi, mask, taupp, a, tw2, tw3, partials, step_ctr, blockIdx.x,
+  plane_body<T, N2, N3, MASK, TAUPP, M>(W, xi, mask, taupp, a, tw2, tw3, partials, step_ctr, blockIdx.x,
                                      reinterpret_cast<cx<T>*>(lds_raw));
 }
 
@@ -1216,6 +1270,7 @@ struct EnsClassDev {
   int* step_ctr;
   long P;
   int n, F, B, S, c3_tile;
+  int lean;                // master in the lean FP32 form
   long plane_first, col_first;  // first item of the class in phase 1 / phase 2
 };
 struct EnsDev {
@@ -1227,12 +1282,22 @@ struct EnsDev {
 
 // (Bodies are not inlined into the persistent kernel, so its registers are
 // those of the largest body rather than of all of them together.)
+template <typename T, int N, typename M>
+__device__ __forceinline__ void ens_plane_m(const EnsClassDev* __restrict__ c, long local, cx<T>* sm, M xi) {
+  plane_body<T, N, N, false, false, M>(reinterpret_cast<cx<T>*>(c->W), xi, nullptr, nullptr, c->sa,
+                                       reinterpret_cast<const cx<T>*>(c->tw[1]),
+                                       reinterpret_cast<const cx<T>*>(c->tw[2]), c->partials, c->step_ctr, local, sm);
+}
 template <typename T, int N>
 __device__ __noinline__ void ens_plane(const EnsClassDev* __restrict__ c, long local, cx<T>* sm) {
-  plane_body<T, N, N, false, false>(reinterpret_cast<cx<T>*>(c->W), make_master<T>(c->xi, (long)c->F * c->P),
-                                    nullptr, nullptr, c->sa,
-                                    reinterpret_cast<const cx<T>*>(c->tw[1]),
-                                    reinterpret_cast<const cx<T>*>(c->tw[2]), c->partials, c->step_ctr, local, sm);
+  const Master<T> m = make_master<T>(c->xi, (long)c->F * c->P, c->lean != 0);
+  if constexpr (sizeof(T) == 8) {
+    ens_plane_m<T, N>(c, local, sm, m);
+  } else if (m.im) {
+    ens_plane_m<T, N>(c, local, sm, MasterF32C{m.r, m.im});
+  } else {
+    ens_plane_m<T, N>(c, local, sm, MasterF32L{m.r});
+  }
 }
 template <typename T, int N>
 __device__ __noinline__ void ens_cols(const EnsClassDev* __restrict__ c, long local, cx<T>* sm) {

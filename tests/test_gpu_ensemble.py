@@ -50,7 +50,7 @@ def oracle_run(w, m, steps, quirk=True):
     return o.get_xi(), g
 
 
-@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED, pa.MIXED_LEAN])
 def test_ensemble_equals_single_contexts(prec):
     """43 steps (five graph replays + three eager steps) of a 6-instance,
     3-class ensemble against each instance run alone: bit-identical."""
@@ -229,7 +229,7 @@ def test_time_step_kernels_follows_the_plan_and_advances_state():
         assert np.array_equal(x, xs), plan
 
 
-@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED, pa.MIXED_LEAN])
 def test_persistent_step_equals_phased(prec):
     """The persistent cooperative step (one launch per step) runs the same
     item code as the per-class kernels: an ensemble stepped both ways, with

@@ -120,7 +120,7 @@ def test_interleaved_frozen_points_and_per_point_tau():
     assert np.array_equal(xg[1, 3:7], w.xi[1, 3:7])
 
 
-@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED])
+@pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED, pa.MIXED_LEAN])
 def test_interleaved_ensemble_equals_single(prec):
     """A batched class in the interleaved layout gives each instance the bits
     of its single-context run with the same plan."""
