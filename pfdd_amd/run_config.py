@@ -5,7 +5,8 @@ mixed-precision error against the FP64 run of the same instance.
     python -m pfdd_amd.run_config c4 [--steps N] [--every 50] [--json out.json]
     python -m pfdd_amd.run_config c3 [--steps N] [--count 512] [--json out.json]
 
-c2: 256^3 x 12, 64 seeded loops: FP64 and mixed (stored and recomputed B) GL
+c2: 256^3 x 12, 64 seeded loops: FP64, mixed (stored and recomputed B) and
+    lean mixed (plain FP32 master, recomputed B) GL
     runs of N steps (default 2000); reports GL steps/s, grid-points x systems
     / s, and the errors of the mixed runs in xi (max abs), slipped area
     (sum_a gamma_a P) and mean stress.
@@ -21,7 +22,7 @@ import time
 
 import numpy as np
 
-from . import (B_RECOMPUTE, B_STORED, ENS_PERSISTENT, ENS_PHASED, FP64, MIXED, Context, Ensemble, PfddError,
+from . import (B_RECOMPUTE, B_STORED, ENS_PERSISTENT, ENS_PHASED, FP64, MIXED, MIXED_LEAN, Context, Ensemble, PfddError,
                describe_plan)
 from . import configs as C
 from . import hip
@@ -154,11 +155,12 @@ def run_ensemble(w, steps, precision, tune_steps=8):
     return out, e
 
 
-def step_bytes(S, P, fp64, stored):
-    """Stage-minimum bytes of a GL step (the mixed master xi takes 12 bytes a point)."""
+def step_bytes(S, P, fp64, stored, lean=False):
+    """Stage-minimum bytes of a GL step (the mixed master xi takes 12 bytes a
+    point, 8 in the lean form)."""
     if fp64:
         return (144 * S + (4 * S * (S + 1) if stored else 0)) * P
-    return (88 * S + (2 * S * (S + 1) if stored else 0)) * P
+    return ((80 if lean else 88) * S + (2 * S * (S + 1) if stored else 0)) * P
 
 
 def main():
@@ -188,6 +190,16 @@ def main():
                                         mean_stress_max_abs_err=float(np.abs(sr - s64).max()),
                                         step_roofline_frac=step_bytes(w.ns, w.P, False, False) * rr["steps_per_s"] / 8e12)
         del xr
+        rl, xl, gal, el, sl = run_plain(w, steps, MIXED_LEAN, B_RECOMPUTE)
+        print(f"[run_config] mixed lean (FP32 master) recomputed B {rl['steps_per_s']:.2f} steps/s", flush=True)
+        res["mixed_lean_recompute_B"] = dict(rl, speedup=rl["steps_per_s"] / r64["steps_per_s"],
+                                             xi_max_abs_err=float(np.abs(xl - x64).max()),
+                                             slipped_area_rel_err=float(abs(gal[-1].sum() - ga64[-1].sum())
+                                                                        / abs(ga64[-1].sum())),
+                                             mean_stress_max_abs_err=float(np.abs(sl - s64).max()),
+                                             step_roofline_frac=step_bytes(w.ns, w.P, False, False, True)
+                                             * rl["steps_per_s"] / 8e12)
+        del xl
         area64, area32 = ga64[-1].sum() * w.P, ga32[-1].sum() * w.P
         res.update({"shape": w.shape, "ns": w.ns, "steps": steps, "fp64": r64, "mixed_stored_B": r32,
                     "speedup": r32["steps_per_s"] / r64["steps_per_s"],
