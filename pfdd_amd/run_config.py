@@ -226,9 +226,22 @@ def main():
             a32, a64 = float(np.sum(x32.real)), float(np.sum(x64.real))   # slipped area, sum_a gamma_a P
             aerr = max(aerr, abs(a32 - a64) / max(abs(a64), 1e-300))
         e32.close()
+        rl, el = run_ensemble(w, steps, MIXED_LEAN)
+        print(f"[run_config] lean mixed {rl['steps_per_s']:.2f} ensemble steps/s "
+              f"({rl['gridpoints_x_systems_per_s']:.3e} pts*sys/s)", flush=True)
+        lxerr, laerr = 0.0, 0.0
+        for i in range(w.count):
+            xl, x64 = el.download_xi(i), e64.download_xi(i)
+            lxerr = max(lxerr, float(np.abs(xl - x64).max()))
+            al, a64 = float(np.sum(xl.real)), float(np.sum(x64.real))
+            laerr = max(laerr, abs(al - a64) / max(abs(a64), 1e-300))
+        el.close()
         e64.close()
         # algorithmic bytes of one ensemble step (recompute B)
         pts = w.total_points()
+        res["mixed_lean"] = dict(rl, speedup=rl["steps_per_s"] / r64["steps_per_s"], xi_max_abs_err=lxerr,
+                                 slipped_area_max_rel_err=laerr,
+                                 step_roofline_frac=step_bytes(3, pts, False, False, True) * rl["steps_per_s"] / 8e12)
         res.update({"count": w.count, "ns": w.ns, "total_points": pts, "steps": steps,
                     "mixed": r32, "fp64": r64, "speedup": r32["steps_per_s"] / r64["steps_per_s"],
                     "xi_max_abs_err": xerr, "slipped_area_max_rel_err": aerr,
