@@ -31,6 +31,31 @@ def test_library_exports_every_declared_function():
     assert sorted(pa.EXPORTED) == names
 
 
+def header_constants():
+    """Every `PFDD_NAME = value` of the header's enums, evaluated."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "pfdd_gpu.h")).read(), flags=re.S)
+    out = {}
+    for name, expr in re.findall(r"\b(PFDD_[A-Z0-9_]+)\s*=\s*(-?[0-9]+(?:\s*<<\s*[0-9]+)?)", txt):
+        out[name[5:]] = eval(expr)
+    return out
+
+
+def test_header_constants_match_package():
+    """The package's constants are the header's (precision modes, B modes,
+    plan bits, stages, ensemble modes, error codes)."""
+    h = header_constants()
+    assert len(h) >= 35
+    checked = 0
+    for name, value in h.items():
+        if hasattr(pa, name):
+            assert getattr(pa, name) == value, name
+            checked += 1
+    for name in ("FP64", "MIXED", "MIXED_LEAN", "B_STORED", "B_RECOMPUTE", "PLAN_INTERLEAVED",
+                 "PLAN_FIELD_MAJOR", "STAGE_FINISH", "ENS_PERSISTENT"):
+        assert name in h and hasattr(pa, name), name
+    assert checked >= 30
+
+
 def test_strerror_names_codes():
     for rc in (pa.OK, pa.E_HIP, pa.E_ALLOC, pa.E_SHAPE, pa.E_RADIX, pa.E_NAN, pa.E_UNSUPPORTED, pa.E_STATE):
         assert pa.strerror(rc) and pa.strerror(rc) != "unknown error"
@@ -58,9 +83,10 @@ def test_create_validates_before_touching_the_device():
     assert _create_rc(p) == pa.E_SHAPE
     assert _create_rc(pa.make_params(**{**base, "C11": float("nan")})) == pa.E_NAN
     assert _create_rc(pa.make_params(**{**base, "dslip": [1.0, 0.0, 1.0]})) == pa.E_NAN
-    p = pa.make_params(**base)
-    p.precision = 7
-    assert _create_rc(p) == pa.E_SHAPE
+    for bad in (7, 3, -1):
+        p = pa.make_params(**base)
+        p.precision = bad
+        assert _create_rc(p) == pa.E_SHAPE
 
 
 @pytest.mark.parametrize("shape,pp", [((32, 2, 32), None), ((128, 128, 128), None), ((16, 12, 8), (3, 5, 6)),
