@@ -10,8 +10,8 @@ c2: 256^3 x 12, 64 seeded loops: FP64, mixed (stored and recomputed B) and
     runs of N steps (default 2000); reports GL steps/s, grid-points x systems
     / s, and the errors of the mixed runs in xi (max abs), slipped area
     (sum_a gamma_a P) and mean stress.
-c4: 512^3 x 3 with the applied-shear ramp 0 -> 0.03 mu: mixed (recompute B)
-    and FP64 runs with <sigma>, <eps> sampled every `every` steps (the
+c4: 512^3 x 3 with the applied-shear ramp 0 -> 0.03 mu: mixed, lean mixed
+    (recompute B) and FP64 runs with <sigma>, <eps> sampled every `every` steps (the
     stress-strain curve), plus mixed with stored B over a shorter window for
     the stored-vs-recompute comparison.
 """
@@ -255,6 +255,15 @@ def main():
         print(f"[run_config] mixed recompute {r32['steps_per_s']:.2f} steps/s", flush=True)
         r64, x64, c64 = run_ramp(w, steps, FP64, B_RECOMPUTE, a.every)
         print(f"[run_config] fp64 recompute {r64['steps_per_s']:.2f} steps/s", flush=True)
+        rl, xl, cl = run_ramp(w, steps, MIXED_LEAN, B_RECOMPUTE, a.every)
+        print(f"[run_config] lean mixed recompute {rl['steps_per_s']:.2f} steps/s", flush=True)
+        res["mixed_lean_recompute"] = dict(
+            rl, speedup=rl["steps_per_s"] / r64["steps_per_s"], xi_max_abs_err=float(np.abs(xl - x64).max()),
+            stress_strain_max_abs_err={
+                "avesigma": float(max(abs(np.array(p["avesigma"]) - np.array(q["avesigma"])).max()
+                                      for p, q in zip(cl, c64))),
+                "aveps": float(max(abs(np.array(p["aveps"]) - np.array(q["aveps"])).max() for p, q in zip(cl, c64)))})
+        del xl
         win = min(steps, 200)
         rs, _, _ = run_ramp(C.c4(B_STORED, steps=steps), win, MIXED, B_STORED, win)
         rr, _, _ = run_ramp(w, win, MIXED, B_RECOMPUTE, win)
