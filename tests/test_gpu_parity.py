@@ -619,6 +619,27 @@ def test_driver_plan_and_autotune_keys(tmp_path, args):
     assert np.abs(xi[:w.ns, ..., 0] + 1j * xi[:w.ns, ..., 1] - xo).max() <= 1e-12
 
 
+@pytest.mark.parametrize("prec", ["mixed", "mixed_lean"])
+def test_driver_mixed_precision_keys(tmp_path, prec):
+    """The driver's precision= key: mixed and lean mixed c0 runs stay within
+    the mixed-precision bound of the oracle (xi <= 1e-4), and the lean run's
+    xi.dat holds FP32 values."""
+    import subprocess
+    say(f"driver c0 steps=40 precision={prec}")
+    exe = os.path.join(os.path.dirname(pa.LIB_PATH), "pfdd")
+    r = subprocess.run([exe, "c0", "steps=40", "fields=0", "energy=0", f"precision={prec}", f"outdir={tmp_path}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    w = C.c0()
+    xi = np.fromfile(tmp_path / "xi.dat", dtype=np.float64).reshape(w.ns + 9, *w.shape, 2)[:w.ns]
+    xo, _, _ = run_oracle(w, 40)
+    got = xi[..., 0] + 1j * xi[..., 1]
+    assert np.abs(got - xo).max() <= 1e-4
+    assert np.abs(got - xo).max() > 0.0
+    if prec == "mixed_lean":
+        assert np.array_equal(xi, xi.astype(np.float32).astype(np.float64))
+
+
 def test_driver_converge_mode_runs():
     import subprocess
     import tempfile
