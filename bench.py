@@ -90,11 +90,20 @@ def step_alg_bytes(S, P, fp64, stored, lean=False):
     return ((80 if lean else 88) * S + (2 * S * (S + 1) if stored else 0)) * P
 
 
+# GL steps per captured hipGraph chunk (GRAPH_STEPS in pfdd_api.hip): the
+# first gl_steps call of at least this many steps captures and instantiates
+# the graph, so the warm-up runs at least that many and the timed region only
+# replays it.
+GRAPH_STEPS = 8
+
+
 def timed_steps(ctx, steps, warmup):
-    ctx.gl_steps(warmup, record=False)
+    """(ms for `steps` GL steps, host wall s, warm-up steps done): the warm-up
+    is at least GRAPH_STEPS steps, so the graph is captured before e0."""
+    wu = max(warmup, GRAPH_STEPS)
+    ctx.gl_steps(wu, record=False)
     ctx.sync()
     e0, e1 = hip.Event(), hip.Event()
-    ctx.sync()
     t0 = time.perf_counter()
     e0.record(ctx.stream)
     ctx.gl_steps(steps, record=False)
@@ -102,7 +111,7 @@ def timed_steps(ctx, steps, warmup):
     e1.synchronize()
     ctx.sync()
     wall = time.perf_counter() - t0
-    return e0.elapsed_ms(e1), wall
+    return e0.elapsed_ms(e1), wall, wu
 
 
 def time_kernels(ctx, nsteps=20):
@@ -143,27 +152,53 @@ def make_ctx(w, precision, bmode):
     return ctx
 
 
-def cpu_baseline(w, budget_s=15.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, budget_s=10.0):
+    """The FP64 oracle on the same instance, on the cores this process may use
+    (at most 16, the host share of one GPU) and on one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oraclepy as O
-    cores = max(1, min(16, os.cpu_count() or 1))
-    p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * w.ns, C.asf2_reference(w.ns), [1.0] * w.ns,
-                      nthreads=cores)
-    o = O.Oracle(p)
-    o.set_xi(w.xi)
-    o.set_tau(w.tau)
-    o.prologue(True)
-    t = time.perf_counter()
-    o.gl_steps(1)
-    t1 = time.perf_counter() - t
-    n = int(max(1, min(100, budget_s / max(t1, 1e-6))))
-    t = time.perf_counter()
-    o.gl_steps(n)
-    dt = time.perf_counter() - t
-    o.close()
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = max(1, min(16, avail))
+
+    def rate(nthreads):
+        p = O.make_params(w.shape, w.eps, C.C11, C.C12, C.C44, [0.1] * w.ns, C.asf2_reference(w.ns),
+                          [1.0] * w.ns, nthreads=nthreads)
+        o = O.Oracle(p)
+        o.set_xi(w.xi)
+        o.set_tau(w.tau)
+        o.prologue(True)
+        t = time.perf_counter()
+        o.gl_steps(1)
+        t1 = time.perf_counter() - t
+        n = int(max(1, min(100, budget_s / max(t1, 1e-6))))
+        t = time.perf_counter()
+        o.gl_steps(n)
+        dt = time.perf_counter() - t
+        o.close()
+        return n, dt
+
+    n, dt = rate(cores)
+    n1, dt1 = rate(1)
     return {"value": n / dt, "unit": "GL steps/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle (oracle/pfdd_oracle.c, OpenMP, own Stockham FFT) on the same 128^3x12 c1 "
-                      f"instance: {n} GL steps after 1 warm-up step, {dt:.1f} s"}
+                      f"instance: {n} GL steps after 1 warm-up step, {dt:.1f} s on {cores} threads",
+            "single_core": {"value": n1 / dt1, "unit": "GL steps/s", "cores": 1,
+                            "sample": f"{n1} GL steps after 1 warm-up step, {dt1:.1f} s"},
+            "host": {"nproc": os.cpu_count(), "affinity_cores": avail, "cpu_model": cpu_model()}}
 
 
 def main():
@@ -190,8 +225,7 @@ def main():
 
     # ---- FP64 (value)
     ctx = make_ctx(w, pa.FP64, bmode)
-    ms64, wall64 = timed_steps(ctx, a.steps, a.warmup)
-    xi64 = ctx.download_xi()
+    ms64, wall64, wu = timed_steps(ctx, a.steps, a.warmup)
     stages = time_kernels(ctx)
     copy_ms = copy_reference(ctx)
     plan = ctx.step_plan()
@@ -218,7 +252,7 @@ def main():
                                                           / (copy_ms / 1e3) / 1e9, 1)}
 
     out = {"metric": METRIC, "value": round(sps64, 2), "unit": "GL steps/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": round(ms64 / a.steps, 5), "higher_is_better": True,
+           "warmup": wu, "ms_per_step": round(ms64 / a.steps, 5), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic: seeded loops (SplitMix64, seed 20261202+1), sigma_ij ~ U[-0.02,0.02]",
            "config": {"workload": "c1: 128^3 grid x 12 FCC slip systems, 16 seeded circular loops r~U[5,20], "
@@ -232,31 +266,47 @@ def main():
            "kernels_fp64": kernels, "wall_s_fp64": round(wall64, 3)}
 
     # ---- mixed precision: compensated FP32 master ("mixed"), plain FP32 master ("mixed_lean")
-    for key, prec, lean in (("mixed", pa.MIXED, False), ("mixed_lean", pa.MIXED_LEAN, True)):
-        if a.no_mixed:
-            break
+    modes = [] if a.no_mixed else [("mixed", pa.MIXED, False), ("mixed_lean", pa.MIXED_LEAN, True)]
+    for key, prec, lean in modes:
         ctxm = make_ctx(w, prec, bmode)
-        ms32, _ = timed_steps(ctxm, a.steps, a.warmup)
-        xi32 = ctxm.download_xi()
+        ms32, _, _ = timed_steps(ctxm, a.steps, a.warmup)
         stages32 = time_kernels(ctxm)
         ctxm.close()
         sps32 = a.steps / (ms32 / 1e3)
-        err_xi = float(np.abs(xi32 - xi64).max())
-        area64 = float(xi64.real.sum())
-        area32 = float(xi32.real.sum())
         out[key] = {"value": round(sps32, 2), "unit": "GL steps/s", "speedup_vs_fp64": round(sps32 / sps64, 3),
+                    "ms_per_step": round(ms32 / a.steps, 5),
                     "master_xi": "plain FP32 complex, 8 B" if lean else "compensated FP32 (hi + lo, im), 12 B",
-                    "xi_max_abs_err_vs_fp64": err_xi,
-                    "slipped_area_rel_err_vs_fp64": abs(area32 - area64) / abs(area64),
-                    "after_steps": a.steps + a.warmup,
                     "step_roofline_frac": round(step_alg_bytes(S, P, False, stored, lean) * sps32 / 1e9
                                                 / HBM_PEAK_GBS, 4),
                     "kernels": {n: {"avg_ms": round(ms, 5), "launches_per_step": k,
                                     "GB_s": round(stage_bytes(st, S, P, ntri, False, stored, lean)
                                                   / (ms / 1e3) / 1e9, 1)}
                                 for n, (st, k, ms) in stages32.items()}}
-        del xi32
-    del xi64
+
+    # ---- errors of the mixed modes against FP64 over the workload's full 1000 GL steps (fresh
+    # contexts, outside the timed region): xi, total slipped area, mean stress
+    if modes:
+        def run_full(prec):
+            c = make_ctx(w, prec, bmode)
+            g, _ = c.gl_steps(w.steps)
+            x = c.download_xi()
+            _, sig = c.strain_stress(w.s_tensor_avepst())
+            c.close()
+            return x, g, sig
+
+        xi64, g64, sig64 = run_full(pa.FP64)
+        area64 = float(xi64.real.sum())
+        for key, prec, _ in modes:
+            xi32, g32, sig32 = run_full(prec)
+            out[key].update({
+                "xi_max_abs_err_vs_fp64": float(np.abs(xi32 - xi64).max()),
+                "slipped_area_rel_err_vs_fp64": abs(float(xi32.real.sum()) - area64) / abs(area64),
+                "gamma_max_rel_err_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
+                "mean_stress_max_abs_err_vs_fp64": float(np.abs(sig32 - sig64).max()),
+                "mean_stress_rel_err_vs_fp64": float(np.abs(sig32 - sig64).max() / np.abs(sig64).max()),
+                "error_after_steps": w.steps})
+            del xi32
+        del xi64
 
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)

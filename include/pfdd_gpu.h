@@ -328,6 +328,13 @@ int pfdd_line_points(int n1, int n2, int n3, const double normal[3], const int p
  * 22, 23, 33) drawn in that order from U[-amp, amp). */
 void pfdd_seed_sigma(double amp, unsigned long long *rng_state, double sigma[3][3]);
 
+/* plasticconverge (G.c:3732-3785), the convergence test of the GL loop:
+ * records step it_plastic in testplastic[] (zeroed by the caller before the
+ * loop), updates the run counter *countgamma, prints the reference's log line
+ * ("(gamma-gammalast)/gamma = ..." or "gamma-gammalast = ...") on stdout
+ * after that update, and returns 0 when the loop has converged, 1 otherwise. */
+int pfdd_plastic_converge(double gamma, double gammalast, int it_plastic, int *testplastic, int *countgamma);
+
 /* ---- the reference's FFT seam on host arrays ----------------------------
  * fft_forward / fft_backward have the exact signature and semantics of
  * G.c:209-273 (in place, unnormalised, batch contiguous N1*N2*N3 complex

@@ -58,6 +58,7 @@ EXPORTED = [
     "pfdd_ensemble_gl_steps", "pfdd_ensemble_autotune", "pfdd_ensemble_set_plan", "pfdd_ensemble_sync",
     "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes", "pfdd_time_step_kernels", "pfdd_ensemble_set_mode",
     "pfdd_ensemble_get_mode", "pfdd_set_line_points", "pfdd_line_sums", "pfdd_gl_steps_lines", "pfdd_line_points",
+    "pfdd_plastic_converge",
 ]
 ENS_PHASED, ENS_PERSISTENT = 0, 1
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
@@ -148,6 +149,7 @@ def load():
     L.pfdd_gl_steps_lines.argtypes = [vp, ci, vp, vp, vp]
     L.pfdd_line_points.argtypes = [ci, ci, ci, vp, vp, vp, ci]
     L.pfdd_set_plan.argtypes = [vp, ci]
+    L.pfdd_plastic_converge.argtypes = [cd, cd, ci, vp, vp]
     L.pfdd_get_plan.argtypes = [vp]
     L.pfdd_autotune.argtypes = [vp, ci, vp, fp]
     L.pfdd_create_ensemble.argtypes = [vp, ci, ctypes.POINTER(vp)]

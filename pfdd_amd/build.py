@@ -1,6 +1,7 @@
 """Build recipe for libpfdd_gpu.so (gfx950) and the pfdd driver, in-tree.
 
     python -m pfdd_amd.build            # or __graft_entry__.build()
+    python -m pfdd_amd.build --incremental   # recompile only objects older than their sources
 
 hipcc cross-compiles for gfx950 without a GPU.  Every HIP compile line carries
 -Xclang -target-feature -Xclang -packed-fp32-ops (packed FP32 ops disabled).
@@ -33,15 +34,30 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
+def _fresh(obj, src):
+    """obj exists and is newer than src and every header it may include."""
+    if not os.path.exists(obj):
+        return False
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith((".hpp", ".h"))] + \
+        [os.path.join(INC, h) for h in os.listdir(INC)]
+    t = os.path.getmtime(obj)
+    return all(os.path.getmtime(d) < t for d in deps)
+
+
+def build(verbose=False, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     for s in C_SOURCES:
         o = os.path.join(BUILD, s + ".o")
-        _run(["gcc", "-O2", "-fPIC", "-std=c99", "-Wall", f"-I{INC}", "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
+        if incremental and _fresh(o, os.path.join(CSRC, s)):
+            continue
+        _run(["gcc", "-O2", "-fPIC", "-std=c99", "-Wall", f"-I{INC}", "-c", os.path.join(CSRC, s), "-o", o])
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s + ".o")
+        if incremental and _fresh(o, os.path.join(CSRC, s)):
+            objs.append(o)
+            continue
         out = _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
         if verbose and out.strip():
             print(out)
@@ -57,4 +73,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -5,6 +5,7 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdio.h>
 
 #include "pfdd_gpu.h"
 
@@ -160,4 +161,34 @@ int pfdd_line_points(int n1, int n2, int n3, const double normal[3], const int p
     }
   }
   return cnt;
+}
+
+/* plasticconverge, G.c:3732-3785.  A step "hits" when the relative change of
+ * gamma is below 1e-8 (or |gamma| < 1e-12; for gamma == 0: |change| < 1e-15
+ * or |gamma| < 1e-14).  From step 29 on, a hit extends the run of hits when
+ * the last 30 steps all hit and resets it otherwise.  The log line is printed
+ * after that update, as in the reference, and the sixth consecutive run ends
+ * the loop (return 0). */
+int pfdd_plastic_converge(double gamma, double gammalast, int it_plastic, int *testplastic, int *countgamma)
+{
+  const int nonzero = gamma != 0;
+  const double d = nonzero ? (gamma - gammalast) / gamma : gamma - gammalast;
+  const int hit = nonzero ? (fabs(d) < 1e-8 || fabs(gamma) < 1e-12) : (fabs(d) < 1e-15 || fabs(gamma) < 1e-14);
+  if (hit) {
+    testplastic[it_plastic] = 1;
+    if (it_plastic >= 29) {
+      int run = 0;
+      for (int i = 0; i < 30; i++) run += testplastic[it_plastic - i];
+      *countgamma = run == 30 ? *countgamma + 1 : 0;
+    }
+    if (nonzero)
+      printf("(gamma-gammalast)/gamma = %e     %d\n", d, *countgamma);
+    else
+      printf("gamma-gammalast = %e     %d\n", d, *countgamma);
+  }
+  if (*countgamma == 5) {
+    (*countgamma)++;
+    return 0;
+  }
+  return 1;
 }

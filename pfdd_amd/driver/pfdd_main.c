@@ -22,7 +22,15 @@
  * 1000 steps), c2 (256^3 x 12, 64 loops, 2000 steps, mixed precision).
  * keys: n1 n2 n3 systems(plane111|fcc) nloops rmin rmax seed sigma_amp tau0
  *       steps nt converge plastic_max precision(fp64|mixed|mixed_lean) bmode(recompute|stored)
- *       quirk pfout_every fields(1|0) energy(1|0) plan autotune outdir
+ *       quirk pfout_every fields(1|0) energy(1|0) plan autotune restart(1|0) xi_in outdir
+ *
+ * Checkpoints (G.c:4361-4382, 4540-4547, 4777-4788): the run ends by writing
+ * xi.dat and xi_bc.dat, and xi1.dat / xi_bc1.dat (the itp == 0 copies), as
+ * raw doubles [NS+9][N1][N2][N3][2]; the last outer step writes FinalXi_w.dat.
+ * restart=1 starts from xi.dat / xi_bc.dat in outdir instead of the seeded
+ * loops (the reference's choice == 2; the applied stress is still drawn from
+ * the seeded stream); xi_in=FILE reads xi alone from FILE at it == 0, as
+ * input_xi_from_file does with FinalXi.dat.
  */
 #include <math.h>
 #include <stdio.h>
@@ -59,6 +67,8 @@ typedef struct {
   int pfout_every;      /* 0: no pfout.dat */
   int fields;           /* 1: strain1D.dat / stress1D.dat per strain call, strain.txt / stress.txt at the end */
   int energy;           /* 1: checkEnergy.dat per outer step, checkEbarrier.dat per GL step */
+  int restart;          /* 1: read xi / xi_bc from xi.dat / xi_bc.dat in outdir (choice == 2, G.c:4366-4382) */
+  char xi_in[512];      /* non-empty: read xi from this file at it == 0 (input_xi_from_file, G.c:3966-3987) */
   char outdir[512];
 } config;
 
@@ -132,6 +142,8 @@ static void set_key(config *c, const char *kv)
   else if (!strcmp(key, "pfout_every")) c->pfout_every = atoi(v);
   else if (!strcmp(key, "fields")) c->fields = atoi(v);
   else if (!strcmp(key, "energy")) c->energy = atoi(v);
+  else if (!strcmp(key, "restart")) c->restart = atoi(v);
+  else if (!strcmp(key, "xi_in")) snprintf(c->xi_in, sizeof(c->xi_in), "%s", v);
   else if (!strcmp(key, "outdir")) snprintf(c->outdir, sizeof(c->outdir), "%s", v);
   else {
     fprintf(stderr, "pfdd: unknown key %s\n", key);
@@ -212,33 +224,30 @@ static void die(int rc, const char *what)
   }
 }
 
-/* plasticconverge, G.c:3732-3785 */
-static int plastic_converge(double gamma, double gammalast, int it_plastic, int *testplastic, int *countgamma)
+/* Reads n doubles of a binary dump ([NSV][N1][N2][N3][2], G.c:4366-4369). */
+static void read_dump(const char *path, double *x, size_t n)
 {
-  int flag = 1;
-  int hit;
-  if (gamma != 0) {
-    double rel = fabs((gamma - gammalast) / gamma);
-    hit = rel < 1e-8 || fabs(gamma) < 1e-12;
-    if (hit) printf("(gamma-gammalast)/gamma = %e     %d\n", (gamma - gammalast) / gamma, *countgamma);
-  } else {
-    hit = fabs(gamma - gammalast) < 1e-15 || fabs(gamma) < 1e-14;
-    if (hit) printf("gamma-gammalast = %e     %d\n", gamma - gammalast, *countgamma);
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    perror(path);
+    exit(1);
   }
-  if (hit) {
-    testplastic[it_plastic] = 1;
-    if (it_plastic >= 29) {
-      int sum = 0;
-      for (int i = 0; i < 30; i++) sum += testplastic[it_plastic - i];
-      if (sum == 30) (*countgamma)++;
-      else *countgamma = 0;
-    }
+  const size_t got = fread(x, sizeof(double), n, f);
+  fclose(f);
+  if (got != n) {
+    fprintf(stderr, "pfdd: %s holds %zu of the %zu doubles of [NS+9][N1][N2][N3][2]\n", path, got, n);
+    exit(1);
   }
-  if (*countgamma == 5) {
-    (*countgamma)++;
-    flag = 0;
+}
+
+static void write_dump(const config *c, const char *name, const double *x, size_t n)
+{
+  FILE *f = open_out(c, name, "wb");
+  if (fwrite(x, sizeof(double), n, f) != n) {
+    fprintf(stderr, "pfdd: short write of %s\n", name);
+    exit(1);
   }
-  return flag;
+  fclose(f);
 }
 
 /* The compliance S_ijkl of avestrain (G.c:652-666), cubic form. */
@@ -373,7 +382,7 @@ int main(int argc, char **argv)
     return 1;
   }
   for (long q = (long)nsys * P; q < (long)nsv * P; q++) xi_bc[2 * q] = 1.0; /* virtual fields frozen */
-  printf(" Generating initial data\n");
+  if (!cfg.restart) printf(" Generating initial data\n"); /* choice == 1 (G.c:4347-4348) */
   double sigma[3][3] = {{0}};
   unsigned long long rng = cfg.seed;
   if (cfg.nloops == 0) {
@@ -393,6 +402,13 @@ int main(int argc, char **argv)
       for (int j = 0; j < 3; j++) sigma[i][j] = cfg.tau0 * se[i][j] / se_e;
   } else {
     pfdd_seed_sigma(cfg.sigma_amp, &rng, sigma);
+  }
+  if (cfg.restart) { /* choice == 2: xi and xi_bc from the dumps of an earlier run (G.c:4366-4369) */
+    char path[1100];
+    snprintf(path, sizeof(path), "%s/xi.dat", cfg.outdir);
+    read_dump(path, xi, (size_t)2 * nsv * P);
+    snprintf(path, sizeof(path), "%s/xi_bc.dat", cfg.outdir);
+    read_dump(path, xi_bc, (size_t)2 * nsv * P);
   }
   double tau[PFDD_MAX_NS];
   pfdd_resolve_tau(nsys, (const double(*)[3])sigma, (const double(*)[3][3])p.eps, tau);
@@ -482,6 +498,10 @@ int main(int argc, char **argv)
         write_1d(of6, sfield, 1.0 / mu, p.n1, p.n2, p.n3);
       }
     }
+    if (it == 0 && cfg.xi_in[0]) { /* input_xi_from_file (G.c:3966-3987): xi only, xi_bc kept */
+      read_dump(cfg.xi_in, xi, (size_t)2 * nsv * P);
+      die(pfdd_upload_xi(ctx, xi, xi_bc, nsv), "pfdd_upload_xi");
+    }
     /* fft_forward + calc_data2_datag + virtualevolv data2 term */
     die(pfdd_begin_outer(ctx, cfg.quirk), "pfdd_begin_outer");
     if (cfg.energy) { /* avestrain with energy_in4 reset (G.c:4439-4442), checkEnergy line (G.c:4530-4536) */
@@ -490,6 +510,10 @@ int main(int argc, char **argv)
       const double e3 = line_e3(&le, nsys, ls);
       e4 = le.e4_inc;
       fprintf(ofE, "%d    %lf   %lf   %lf   %lf   %lf   %lf\n", it, 0.0, 0.0, e3, e4, 0.0, e3 + e4);
+    }
+    if (it == cfg.nt - 1) { /* FinalXi_w.dat (G.c:4540-4547) */
+      die(pfdd_download_xi(ctx, xi, nsv), "pfdd_download_xi");
+      write_dump(&cfg, "FinalXi_w.dat", xi, (size_t)2 * nsv * P);
     }
     printf("vflag %d, time %d \n", 0, it);
     if (it != 0 && it != cfg.nt - 1) {
@@ -522,7 +546,7 @@ int main(int argc, char **argv)
           else die(pfdd_gl_steps(ctx, 1, gam, gam_a), "pfdd_gl_steps");
           gamma = gam[0];
           for (int a = 0; a < nsys; a++) gamma1[a] = gam_a[a];
-          check = plastic_converge(gamma, gammalast, it_plastic, testplastic, &countgamma);
+          check = pfdd_plastic_converge(gamma, gammalast, it_plastic, testplastic, &countgamma);
           printf("in evolve plastic:    %d    %d\n", it_plastic, check);
           if (cfg.energy && check) { /* only while plasticconverge says go on (G.c:4589, 4646-4650) */
             const double e3 = line_e3(&le, nsys, lsum);
@@ -548,14 +572,13 @@ int main(int argc, char **argv)
       fprintf(of2, " %e \n", gamma);
     }
   }
-  /* binary dumps xi.dat / xi_bc.dat, [NSV][N1][N2][N3][2] doubles (G.c:4777-4788) */
+  /* binary dumps xi.dat / xi_bc.dat and, for itp == 0, xi1.dat / xi_bc1.dat,
+   * [NSV][N1][N2][N3][2] doubles (G.c:4777-4788) */
   die(pfdd_download_xi(ctx, xi, nsv), "pfdd_download_xi");
-  FILE *fx = open_out(&cfg, "xi.dat", "wb");
-  FILE *fb = open_out(&cfg, "xi_bc.dat", "wb");
-  fwrite(xi, sizeof(double), (size_t)2 * nsv * P, fx);
-  fwrite(xi_bc, sizeof(double), (size_t)2 * nsv * P, fb);
-  fclose(fx);
-  fclose(fb);
+  write_dump(&cfg, "xi.dat", xi, (size_t)2 * nsv * P);
+  write_dump(&cfg, "xi_bc.dat", xi_bc, (size_t)2 * nsv * P);
+  write_dump(&cfg, "xi1.dat", xi, (size_t)2 * nsv * P);
+  write_dump(&cfg, "xi_bc1.dat", xi_bc, (size_t)2 * nsv * P);
   fclose(of2);
   fclose(ofpfv);
   if (ofpfout) fclose(ofpfout);

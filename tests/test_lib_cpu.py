@@ -192,9 +192,91 @@ def test_driver_rejects_bad_arguments():
     assert r.returncode != 0 and "pfdd_create" in r.stderr
 
 
+def test_driver_restart_rejects_a_short_dump(tmp_path):
+    """restart=1 reads xi.dat / xi_bc.dat before any device work and refuses a
+    dump shorter than [NS+9][N1][N2][N3][2] doubles."""
+    import subprocess
+    exe = os.path.join(ROOT, "pfdd_amd", "pfdd")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    np.zeros(10).tofile(tmp_path / "xi.dat")
+    np.zeros(10).tofile(tmp_path / "xi_bc.dat")
+    r = subprocess.run([exe, "c0", "steps=2", f"outdir={tmp_path}", "restart=1"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "holds 10 of the 786432 doubles" in r.stderr
+
+
 def test_fft_seam_requires_init():
     L = pa.load()
     L.pfdd_fft_host_release()
     a = np.zeros(8, dtype=np.complex64)
     L.fft_forward(a.ctypes.data, 1)
     assert L.pfdd_fft_host_status() == pa.E_STATE
+
+
+def _captured_stdout(fn):
+    """Runs fn() with the C-level stdout (fd 1) sent to a temporary file and
+    returns (fn's result, the bytes written)."""
+    import sys
+    import tempfile
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved = os.dup(1)
+    with tempfile.TemporaryFile() as tmp:
+        os.dup2(tmp.fileno(), 1)
+        try:
+            out = fn()
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        return out, tmp.read()
+
+
+def _gamma_sequences():
+    """gamma sequences that take plasticconverge through its branches: a
+    converging series with one outlier (the run counter resets), a series
+    that converges for good (the loop ends), gamma == 0 throughout, and
+    gamma crossing zero."""
+    s = np.arange(400)
+    conv = 0.37 + 1e-2 * 0.5 ** (s / 3.0)
+    outlier = conv.copy()
+    outlier[70] += 1e-6
+    zero = np.zeros(60)
+    cross = np.concatenate([np.linspace(1e-3, 0.0, 20), np.zeros(40), np.full(40, 1e-13)])
+    return [conv, outlier, zero, cross]
+
+
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref not built")
+def test_plastic_converge_matches_reference_log():
+    """pfdd_plastic_converge (used by the driver) against the compiled
+    plasticconverge (G.c:3732-3785): the same flags, counters and marks, and
+    byte-identical log lines, with the line printed after the counter update."""
+    L = pa.load()
+    ref = R.lib()
+    ref.plasticconverge.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    ref.plasticconverge.restype = ctypes.c_int
+    ends = 0
+    for seq in _gamma_sequences():
+        tp = [np.zeros(len(seq), dtype=np.int32) for _ in range(2)]
+        cnt = [ctypes.c_int(0), ctypes.c_int(0)]
+        logs = [b"", b""]
+        last = 0.0
+        for it, g in enumerate(seq):
+            flags = []
+            for k, fn in enumerate((ref.plasticconverge, L.pfdd_plastic_converge)):
+                f, out = _captured_stdout(lambda: fn(float(g), last, it, tp[k].ctypes.data, ctypes.byref(cnt[k])))
+                flags.append(f)
+                logs[k] += out
+            assert flags[0] == flags[1], (it, g)
+            assert cnt[0].value == cnt[1].value, (it, g)
+            last = float(g)
+            if flags[0] == 0:
+                ends += 1
+                break
+        assert np.array_equal(tp[0], tp[1])
+        assert logs[0] == logs[1]
+        assert logs[0].count(b"\n") > 10
+    assert ends >= 2
