@@ -260,14 +260,19 @@ int pfdd_ensemble_gl_steps(pfdd_ens *e, int nsteps, double *gamma);
 int pfdd_ensemble_autotune(pfdd_ens *e, int steps, int *plans, float *ms_per_step);
 int pfdd_ensemble_set_plan(pfdd_ens *e, int cls, int plan);
 /* How a GL step of the ensemble runs (no reference counterpart):
- * PFDD_ENS_PHASED: every class's kernels, GRAPH_STEPS steps of all classes
- * captured in one hipGraph; PFDD_ENS_PERSISTENT: one persistent cooperative
- * launch per step and precision that takes the plane items of every class
- * from a work counter, then (after a grid barrier) the gamma finish and the
- * fused column items.  Persistent mode needs every class cubic with
- * N in {32,48,64,96}, ns <= 4, recomputed B, no frozen points and uniform
- * shear (PFDD_E_UNSUPPORTED otherwise); it sets each class's plan to
+ * PFDD_ENS_PHASED: every class's kernels; PFDD_ENS_PERSISTENT: one
+ * persistent launch per step and precision whose blocks draw work items from
+ * a counter: the plane items of every class, then each class's gamma finish
+ * and fused column items, each of which waits until its class's plane items
+ * have completed (a per-class completion counter, no grid-wide barrier).  In
+ * both modes GRAPH_STEPS steps of the whole ensemble are one captured
+ * hipGraph.  Persistent mode needs every class cubic with N in
+ * {32,48,64,96}, ns <= 4, recomputed B, no frozen points and uniform shear
+ * (PFDD_E_UNSUPPORTED otherwise); it sets each class's plan to
  * PFDD_PLAN_PLANE | PFDD_PLAN_FUSED_COLS.  Both modes give the same bits.
+ * A completion wait is bounded (about 2 s); one that runs out skips its item
+ * and makes the next synchronous call (pfdd_ensemble_gl_steps with gamma,
+ * pfdd_ensemble_sync) return PFDD_E_HIP.
  * pfdd_ensemble_begin_outer must run again after a change.
  * pfdd_ensemble_autotune tunes the class plans, then times both modes and
  * keeps the faster; pfdd_ensemble_get_mode returns the mode and the two

@@ -1278,11 +1278,20 @@ int begin_outer(pfdd_ctx* c, int quirk) {
 
 // Runs nsteps GL steps of every context in cs (all on one stream), in chunks
 // of up to HIST_CAP steps; GRAPH_STEPS steps of all of them are captured once
-// into *graph and replayed.  After each chunk, sink(ctx index, first step,
-// steps, history rows) receives the gamma history of each context.
+// into *graph and replayed.  step_all (when given) enqueues one step of all
+// of them instead of step() per context.  After each chunk, sink(ctx index,
+// first step, steps, history rows) receives the gamma history of each context.
 template <typename Sink>
 int run_steps(pfdd_ctx* const* cs, int nc, hipStream_t st, hipGraphExec_t* graph, int nsteps, bool want_hist,
               Sink&& sink, const std::function<int()>* step_all = nullptr) {
+  auto one = [&]() -> int {
+    if (step_all) return (*step_all)();
+    for (int i = 0; i < nc; i++) {
+      const int rc = step(cs[i]);
+      if (rc) return rc;
+    }
+    return PFDD_OK;
+  };
   int done = 0;
   std::vector<double> h;
   while (done < nsteps) {
@@ -1291,35 +1300,28 @@ int run_steps(pfdd_ctx* const* cs, int nc, hipStream_t st, hipGraphExec_t* graph
       hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, cs[i]->step_ctr, 0);
     HIPCK(hipPeekAtLastError());
     int left = chunk;
-    // (a step_all, one cooperative launch per step, is not captured)
-    if (!step_all && left >= GRAPH_STEPS && !*graph) {
+    if (left >= GRAPH_STEPS && !*graph) {
       hipGraph_t g;
       HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       int rc = PFDD_OK;
-      for (int s = 0; s < GRAPH_STEPS && rc == PFDD_OK; s++)
-        for (int i = 0; i < nc && rc == PFDD_OK; i++) rc = step(cs[i]);
+      for (int s = 0; s < GRAPH_STEPS && rc == PFDD_OK; s++) rc = one();
       hipError_t ec = hipStreamEndCapture(st, &g);
-      if (rc) return rc;
+      if (rc) {
+        if (ec == hipSuccess) (void)hipGraphDestroy(g);
+        return rc;
+      }
       HIPCK(ec);
       ec = hipGraphInstantiate(graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIPCK(ec);
     }
-    while (!step_all && left >= GRAPH_STEPS && *graph) {
+    while (left >= GRAPH_STEPS && *graph) {
       HIPCK(hipGraphLaunch(*graph, st));
       left -= GRAPH_STEPS;
     }
-    while (left > 0) {
-      if (step_all) {
-        const int rc = (*step_all)();
-        if (rc) return rc;
-      } else {
-        for (int i = 0; i < nc; i++) {
-          const int rc = step(cs[i]);
-          if (rc) return rc;
-        }
-      }
-      left--;
+    for (; left > 0; left--) {
+      const int rc = one();
+      if (rc) return rc;
     }
     if (want_hist) {
       for (int i = 0; i < nc; i++) {
@@ -1850,11 +1852,9 @@ const void* ens_kernel() {
 
 // Work lists and launch shape of the persistent step, per precision.
 int pers_build(pfdd_ens* e) {
-  int dev = 0, ncu = 0, coop = 0;
+  int dev = 0, ncu = 0;
   HIPCK(hipGetDevice(&dev));
   HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-  if (!coop) return PFDD_E_UNSUPPORTED;
   for (int fp = 0; fp < 2; fp++) {
     std::vector<pfdd_ctx*> cl;
     for (pfdd_ctx* c : e->cls)
@@ -1867,7 +1867,7 @@ int pers_build(pfdd_ens* e) {
     std::stable_sort(cl.begin(), cl.end(), [](const pfdd_ctx* a, const pfdd_ctx* b) { return a->p.n1 > b->p.n1; });
     EnsDev h{};
     h.nc = (int)cl.size();
-    long plane = 0, col = h.nc;
+    long plane = 0, col = 0;
     size_t lds = 0;
     for (int k = 0; k < h.nc; k++) {
       const pfdd_ctx* c = cl[k];
@@ -1890,15 +1890,17 @@ int pers_build(pfdd_ens* e) {
       d.c3_tile = c->c3_tile;
       d.plane_first = plane;
       d.col_first = col;
-      plane += (long)c->F * c->p.n1;
+      d.n_planes = (long)c->F * c->p.n1;
+      plane += d.n_planes;
       col += (long)c->B * ((long)c->p.n2 * c->p.n3 / c->c3_tile);
       lds = std::max(lds, std::max(plane_lds_of(c->p.n1, fp), std::max(c->c3_lds, sizeof(double) * c->F)));
     }
+    if (plane + h.nc + col >= (1L << 31)) return PFDD_E_UNSUPPORTED;  // items are drawn as int
     h.n_plane = plane;
-    h.n_col = col;
+    h.n_items = plane + h.nc + col;
     if (!g.q) {
-      if (hipMalloc(&g.q, sizeof(int) * 2) != hipSuccess) return PFDD_E_ALLOC;
-      HIPCK(hipMemsetAsync(g.q, 0, sizeof(int) * 2, e->stream));
+      if (hipMalloc(&g.q, sizeof(int) * ENS_Q_INTS) != hipSuccess) return PFDD_E_ALLOC;
+      HIPCK(hipMemsetAsync(g.q, 0, sizeof(int) * ENS_Q_INTS, e->stream));
     }
     h.q = g.q;
     if (!g.dev && hipMalloc(&g.dev, sizeof(EnsDev)) != hipSuccess) return PFDD_E_ALLOC;
@@ -1916,17 +1918,37 @@ int pers_build(pfdd_ens* e) {
   return PFDD_OK;
 }
 
-// One GL step of every class: one cooperative launch per precision.
+// One GL step of every class: per precision, the counter reset and one launch
+// of k_ens_step (both ordinary kernels, so the step can be captured).
 int pers_step(pfdd_ens* e) {
   for (int fp = 0; fp < 2; fp++) {
     PersGroup& g = e->pg[fp];
     if (!g.nc) continue;
-    void* args[] = {&g.dev};
-    const void* kern = fp ? ens_kernel<double>() : ens_kernel<float>();
-    HIPCK(hipLaunchCooperativeKernel(kern, dim3((unsigned)g.grid), dim3(ENS_THREADS), args, (unsigned)g.lds,
-                                     e->stream));
+    hipLaunchKernelGGL(k_ens_reset, dim3(1), dim3(64), 0, e->stream, g.q);
+    if (fp)
+      hipLaunchKernelGGL(k_ens_step<double>, dim3((unsigned)g.grid), dim3(ENS_THREADS), g.lds, e->stream, g.dev);
+    else
+      hipLaunchKernelGGL(k_ens_step<float>, dim3((unsigned)g.grid), dim3(ENS_THREADS), g.lds, e->stream, g.dev);
   }
-  return PFDD_OK;
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
+// Error flags of the persistent step (a completion wait that ran out); the
+// flags are cleared.  Synchronous.
+int pers_check(pfdd_ens* e) {
+  int bad = 0;
+  for (PersGroup& g : e->pg) {
+    if (!g.nc || !g.q) continue;
+    int flag = 0;
+    HIPCK(hipMemcpyAsync(&flag, g.q + 1, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCK(hipStreamSynchronize(e->stream));
+    if (flag) {
+      bad = 1;
+      hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, e->stream, g.q + 1, 0);
+      HIPCK(hipStreamSynchronize(e->stream));
+    }
+  }
+  return bad ? PFDD_E_HIP : PFDD_OK;
 }
 
 // Plans the persistent step needs (plane kernel + fused columns).
@@ -2046,8 +2068,9 @@ int pfdd_ensemble_gl_steps(pfdd_ens* e, int nsteps, double* gamma) {
   std::vector<std::vector<int>> members(e->cls.size());
   for (int i = 0; i < e->count; i++) members[e->cls_of[i]].push_back(i);
   const int count = e->count;
+  const bool pers = e->mode == PFDD_ENS_PERSISTENT;
   std::function<int()> pstep;
-  if (e->mode == PFDD_ENS_PERSISTENT) {
+  if (pers) {
     for (pfdd_ctx* c : e->cls)
       if (!pers_eligible(c)) return PFDD_E_UNSUPPORTED;  // e.g. a per-point shear set since
     if (!e->pers_built) {
@@ -2056,15 +2079,19 @@ int pfdd_ensemble_gl_steps(pfdd_ens* e, int nsteps, double* gamma) {
     }
     pstep = [e]() { return pers_step(e); };
   }
-  return run_steps(e->cls.data(), (int)e->cls.size(), e->stream, &e->graph, nsteps, gamma != nullptr,
-                   [&](int k, int first, int n, const double* h) {
-                     const int S = e->cls[k]->S;
-                     const size_t row = (size_t)e->cls[k]->B * (S + 1);
-                     for (int s = 0; s < n; s++)
-                       for (size_t b = 0; b < members[k].size(); b++)
-                         gamma[(size_t)(first + s) * count + members[k][b]] = h[(size_t)s * row + b * (S + 1) + S];
-                   },
-                   e->mode == PFDD_ENS_PERSISTENT ? &pstep : nullptr);
+  int rc = run_steps(e->cls.data(), (int)e->cls.size(), e->stream, &e->graph, nsteps, gamma != nullptr,
+                     [&](int k, int first, int n, const double* h) {
+                       const int S = e->cls[k]->S;
+                       const size_t row = (size_t)e->cls[k]->B * (S + 1);
+                       for (int s = 0; s < n; s++)
+                         for (size_t b = 0; b < members[k].size(); b++)
+                           gamma[(size_t)(first + s) * count + members[k][b]] = h[(size_t)s * row + b * (S + 1) + S];
+                     },
+                     pers ? &pstep : nullptr);
+  // a completion wait of the persistent step that ran out shows at the next
+  // synchronous call (with gamma, at once)
+  if (rc == PFDD_OK && pers && gamma) rc = pers_check(e);
+  return rc;
 }
 
 int pfdd_ensemble_set_mode(pfdd_ens* e, int mode) {
@@ -2108,10 +2135,6 @@ int pfdd_ensemble_autotune(pfdd_ens* e, int steps, int* plans, float* ms_per_ste
   bool elig = true;
   for (pfdd_ctx* c : e->cls) elig = elig && pers_eligible(c);
   if (!elig) return PFDD_OK;
-  int coop = 0, dev = 0;
-  HIPCK(hipGetDevice(&dev));
-  HIPCK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-  if (!coop) return PFDD_OK;
   std::vector<void*> xsave(e->cls.size(), nullptr);
   std::vector<int*> csave(e->cls.size(), nullptr);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2140,6 +2163,7 @@ int pfdd_ensemble_autotune(pfdd_ens* e, int steps, int* plans, float* ms_per_ste
     if ((r = pfdd_ensemble_gl_steps(e, timed, nullptr))) return r;
     HIPCK(hipEventRecord(e1, e->stream));
     HIPCK(hipEventSynchronize(e1));
+    if (mode == PFDD_ENS_PERSISTENT && (r = pers_check(e))) return r;
     float ms = 0.f;
     HIPCK(hipEventElapsedTime(&ms, e0, e1));
     ms /= (float)timed;
@@ -2203,7 +2227,7 @@ int pfdd_ensemble_set_plan(pfdd_ens* e, int cls, int plan) {
 int pfdd_ensemble_sync(pfdd_ens* e) {
   if (!e) return PFDD_E_SHAPE;
   HIPCK(hipStreamSynchronize(e->stream));
-  return PFDD_OK;
+  return e->pers_built ? pers_check(e) : PFDD_OK;
 }
 
 void* pfdd_ensemble_stream(pfdd_ens* e) { return e ? (void*)e->stream : nullptr; }

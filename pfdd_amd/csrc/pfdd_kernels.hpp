@@ -15,8 +15,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "pfdd_fft.hpp"
 
 namespace pfdd {
@@ -1250,15 +1248,25 @@ k_strain_stress(const cx<T>* __restrict__ E, Master<T> xi, const ElasticConsts* 
 }
 
 // ---------------------------------------------------------------- persistent ensemble step
-// One cooperative launch runs a whole GL step of every class of an ensemble
-// (of one precision): phase 1 takes plane items (plane_body) of all classes
-// from a work counter, a grid barrier follows, and phase 2 takes the gamma
-// finish of each class and then the fused column items (col_contract_body).
-// The items run exactly the code of the plane / fused column / finish
-// kernels, so a step gives the same bits as those kernels launched per
-// class.  Supported classes: cubic grids N in {32, 48, 64, 96}, S <= 4,
-// recomputed B, no frozen points, uniform resolved shear.
+// One ordinary launch runs a whole GL step of every class of an ensemble (of
+// one precision).  Blocks draw items from one work counter: first the plane
+// items (plane_body) of every class, larger grids first; then the gamma
+// finish of each class; then the fused column items (col_contract_body).  An
+// item of the second kind waits until every plane item of its class has
+// completed (a per-class completion counter): since all plane items are drawn
+// before any other item, the blocks it waits for are already running and
+// never wait themselves, so no grid-wide barrier or co-residency guarantee is
+// needed.  Each wait is bounded; a wait that runs out records an error and
+// the item is skipped (the host reports it).  The items run exactly the code
+// of the plane / fused column / finish kernels, so a step gives the same bits
+// as those kernels launched per class.  Supported classes: cubic grids N in
+// {32, 48, 64, 96}, S <= 4, recomputed B, no frozen points, uniform shear.
 constexpr int ENS_THREADS = 1024, ENS_MAX_CLASSES = 8;
+// q: [0] work counter, [1] error flag (sticky: set by a wait that ran out,
+// cleared by the host), [2 + k] completed plane items of class k
+constexpr int ENS_Q_INTS = 2 + ENS_MAX_CLASSES;
+// spins of at most ~64 clocks each before a wait gives up (~2 s at 2.4 GHz)
+constexpr long ENS_SPIN_LIMIT = 1L << 26;
 struct EnsClassDev {
   void* W;                 // cx<T> [B*S][P]
   void* xi;                // the master (Master<T> over B*S*P points)
@@ -1271,14 +1279,19 @@ struct EnsClassDev {
   long P;
   int n, F, B, S, c3_tile;
   int lean;                // master in the lean FP32 form
-  long plane_first, col_first;  // first item of the class in phase 1 / phase 2
+  long plane_first, col_first;  // first item of the class among the plane / column items
+  long n_planes;           // plane items of the class (F * n)
 };
 struct EnsDev {
-  int* q;                  // work counters of the two phases (zero between launches)
+  int* q;                  // ENS_Q_INTS counters; all but the error flag are zeroed before every launch
   int nc;
-  long n_plane, n_col;     // items per phase; phase 2 starts with the nc finish items
+  long n_plane, n_items;   // plane items; all items (planes, nc finishes, column items)
   EnsClassDev cls[ENS_MAX_CLASSES];
 };
+
+__global__ void k_ens_reset(int* q) {
+  if (threadIdx.x < ENS_Q_INTS && threadIdx.x != 1) q[threadIdx.x] = 0;
+}
 
 // (Bodies are not inlined into the persistent kernel, so its registers are
 // those of the largest body rather than of all of them together.)
@@ -1314,43 +1327,68 @@ __global__ void __launch_bounds__(ENS_THREADS) k_ens_step(const EnsDev* __restri
   extern __shared__ unsigned char lds_raw[];
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   __shared__ int slot;
-  // phase 2's counter is idle until the barrier: clear it now
-  if (blockIdx.x == 0 && threadIdx.x == 0) d->q[1] = 0;
+  __shared__ int ready;        // bit k: this block has seen class k's planes complete
+  __shared__ int go;
+  int* const q = d->q;
+  if (threadIdx.x == 0) ready = 0;
   for (;;) {
     __syncthreads();  // the previous item is done with LDS and `slot`
-    if (threadIdx.x == 0) slot = atomicAdd(&d->q[0], 1);
+    if (threadIdx.x == 0) slot = atomicAdd(&q[0], 1);
     __syncthreads();
     const long it = slot;
-    if (it >= d->n_plane) break;
-    int k = 0;
-    while (k + 1 < d->nc && it >= d->cls[k + 1].plane_first) k++;
-    const EnsClassDev* c = &d->cls[k];
-    const long local = it - c->plane_first;
-    switch (c->n) {
-      case 32: ens_plane<T, 32>(c, local, sm); break;
-      case 48: ens_plane<T, 48>(c, local, sm); break;
-      case 64: ens_plane<T, 64>(c, local, sm); break;
-      case 96: ens_plane<T, 96>(c, local, sm); break;
-      default: break;
-    }
-  }
-  cooperative_groups::this_grid().sync();
-  // every block has drawn its last phase-1 item: clear that counter for the next launch
-  if (blockIdx.x == 0 && threadIdx.x == 0) d->q[0] = 0;
-  for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) slot = atomicAdd(&d->q[1], 1);
-    __syncthreads();
-    const long it = slot;
-    if (it >= d->n_col) break;
-    if (it < d->nc) {
-      ens_finish(&d->cls[it], reinterpret_cast<double*>(lds_raw));
+    if (it >= d->n_items) break;
+    if (it < d->n_plane) {
+      int k = 0;
+      while (k + 1 < d->nc && it >= d->cls[k + 1].plane_first) k++;
+      const EnsClassDev* c = &d->cls[k];
+      const long local = it - c->plane_first;
+      switch (c->n) {
+        case 32: ens_plane<T, 32>(c, local, sm); break;
+        case 48: ens_plane<T, 48>(c, local, sm); break;
+        case 64: ens_plane<T, 64>(c, local, sm); break;
+        case 96: ens_plane<T, 96>(c, local, sm); break;
+        default: break;
+      }
+      __syncthreads();  // every wave's stores of the plane are done
+      if (threadIdx.x == 0) {
+        __threadfence();  // ... and visible device-wide before the count
+        atomicAdd(&q[2 + k], 1);
+      }
       continue;
     }
+    // a finish or column item of class k: wait for the class's plane items
+    const long r = it - d->n_plane;
     int k = 0;
-    while (k + 1 < d->nc && it >= d->cls[k + 1].col_first) k++;
+    if (r < d->nc) {
+      k = (int)r;
+    } else {
+      while (k + 1 < d->nc && r >= d->cls[k + 1].col_first + d->nc) k++;
+    }
     const EnsClassDev* c = &d->cls[k];
-    const long local = it - c->col_first;
+    if (threadIdx.x == 0) {
+      int ok = 1;
+      if (!(ready & (1 << k))) {
+        long spins = 0;
+        while (__hip_atomic_load(&q[2 + k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (int)c->n_planes) {
+          if (++spins > ENS_SPIN_LIMIT) {
+            ok = 0;
+            atomicOr(&q[1], 1);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+        if (ok) ready |= 1 << k;
+      }
+      go = ok;
+    }
+    __syncthreads();
+    if (!go) continue;
+    if (r < d->nc) {
+      ens_finish(c, reinterpret_cast<double*>(lds_raw));
+      continue;
+    }
+    const long local = r - d->nc - c->col_first;
     switch (c->n) {
       case 32: ens_cols<T, 32>(c, local, sm); break;
       case 48: ens_cols<T, 48>(c, local, sm); break;

@@ -231,7 +231,7 @@ def test_time_step_kernels_follows_the_plan_and_advances_state():
 
 @pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED, pa.MIXED_LEAN])
 def test_persistent_step_equals_phased(prec):
-    """The persistent cooperative step (one launch per step) runs the same
+    """The persistent step (one launch per step) runs the same
     item code as the per-class kernels: an ensemble stepped both ways, with
     the plans the persistent step uses, agrees bit for bit (xi and gamma)."""
     w = small_ensemble(count=5, sizes=(32, 48, 64), first=40)
