@@ -85,6 +85,7 @@ struct pfdd_ctx {
   int c3_tile = 0;           // columns per block of the fused column kernel (0: not fused)
   int plane = 0;             // 1: the LDS-resident plane kernel does the N2/N3 passes and the update
   int il = 0;                // 1: W interleaved, [B][P][S] (fields of a point together); else [B][S][P]
+  int il_fixed = 0;          // 1: the interleaved column kernel with compile-time S and tile (k_col_il_fixed)
   size_t c3_lds = 0;         // its dynamic LDS bytes
   int plan = 0;              // the applied PFDD_PLAN_* word
   int r4 = 0;                // bit ax: radix-4 sequence for the standalone passes along axis ax
@@ -444,9 +445,34 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
   return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
 }
 
+// Tile of the compile-time interleaved column kernel (k_col_il_fixed) for
+// ns fields along n1, or 0 where it is not instantiated.
+int il_fixed_tile(int S, int n1, bool fp64) {
+  if (S != 12) return 0;
+  if (n1 == 128) return fp64 ? 2 : 4;
+  if (n1 == 256) return fp64 ? 1 : 2;
+  return 0;
+}
+template <typename T, int N, int TC>
+void k3il_fixed(pfdd_ctx* c, int quirk) {
+  const long nb = ((long)c->p.n2 * c->p.n3) / TC;
+  hipLaunchKernelGGL((k_col_il_fixed<T, N, 12, TC>), dim3((unsigned)nb, (unsigned)c->B),
+                     dim3(ColIlFixed<T, N, 12, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P, quirk,
+                     tw_of<T>(c->tw[0]));
+}
+
 // Column kernel, interleaved layout (c->c3_tile columns of every field per block).
 template <typename T, int N>
 int k3il_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
+  if (!attr_only && c->il_fixed) {
+    if constexpr (N == 128) {
+      k3il_fixed<T, 128, sizeof(T) == 8 ? 2 : 4>(c, quirk);
+    } else if constexpr (N == 256) {
+      k3il_fixed<T, 256, sizeof(T) == 8 ? 1 : 2>(c, quirk);
+    }
+    tap(c, PFDD_STAGE_CONTRACT);
+    return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+  }
   const long nb = ((long)c->p.n2 * c->p.n3) / c->c3_tile;
   const int nt = il_threads<N>(c->S * c->c3_tile);
   const size_t lds = col_il_lds_bytes<T, N>(c->S, c->c3_tile);
@@ -835,6 +861,8 @@ int il_tiles(pfdd_ctx* c, int plan, int& row_tile, int& c3_tile, size_t& c3_lds)
     if (creq) {
       t = 1 << (creq - 1);
       if (!col_fits(t)) return PFDD_E_SHAPE;
+    } else if (const int ft = il_fixed_tile(S, p.n1, c->fp64 != 0); ft > 0 && getenv("PFDD_IL_FIXED")) {
+      t = ft;
     } else {
       while (col_fits(t * 2) && col_lds(t * 2) <= IL_AUTO_LDS && (size_t)t * 2 * S * csz <= 256) t *= 2;
       if (!col_fits(t)) t = 0;
@@ -930,6 +958,7 @@ int apply_plan(pfdd_ctx* c, int plan) {
   c->c3_lds = c3_lds;
   c->plane = plane;
   c->il = il;
+  c->il_fixed = il && c3_tile > 0 && c3_tile == il_fixed_tile(S, p.n1, c->fp64 != 0) && getenv("PFDD_IL_FIXED");
   int rc = PFDD_OK;
   if (il && (rc = c->fp64 ? k1il_attr<double>(c) : k1il_attr<float>(c))) return rc;
   if (c3_tile > 0 && (rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true))) return rc;
@@ -1052,13 +1081,22 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
       }
       h.Tsum[a] = ts;
     }
+    {  // six symmetric components (00, 01, 02, 11, 12, 22) of s_b and of w_t sym(e_a)_t / mu
+      const int ti[6] = {0, 0, 0, 1, 1, 2}, tj[6] = {0, 1, 2, 1, 2, 2};
+      for (int a = 0; a < S; a++)
+        for (int t = 0; t < 6; t++) {
+          const int i = ti[t], j = tj[t];
+          h.sig6[a][t] = 0.5 * (h.sig[a][3 * i + j] + h.sig[a][3 * j + i]);
+          h.e6m[a][t] = (i == j ? 1.0 : 2.0) * 0.5 * (h.eps[a][3 * i + j] + h.eps[a][3 * j + i]) * h.inv_mu;
+          h.sig6f[a][t] = (float)h.sig6[a][t];
+          h.e6mf[a][t] = (float)h.e6m[a][t];
+        }
+    }
     for (int a = 0; a < S; a++)
       for (int b = 0; b < S; b++) {
         double se = 0.0;
         for (int mn = 0; mn < 9; mn++) se += h.sig[a][mn] * h.eps[b][mn];
         h.SE[a][b] = se;
-        h.SEm[a][b] = se * h.inv_mu;
-        h.SEmf[a][b] = (float)h.SEm[a][b];
       }
     h.d[0] = p->d1; h.d[1] = p->d2; h.d[2] = p->d3;
     // frec (G.c:2698-2741) per axis, evaluated with the same arithmetic as the reference

@@ -120,12 +120,17 @@ struct ElasticConsts {
   double mu, ll, mup, inv_mu;      // C44, C12, C11-C12-2C44 (Bmatrix G.c:345-347)
   double sig[MAXS][9];             // s_a = C:e_a
   double SE[MAXS][MAXS];           // s_a:e_b
-  double SEm[MAXS][MAXS];          // s_a:e_b / mu
-  float SEmf[MAXS][MAXS];          // s_a:e_b / mu rounded to FP32
   double Tsum[MAXS];               // sum_ij s_a,ij
   double T[9];                     // T_pq = sum_ij C_pqij
   double C[81];                    // C_ijkl
   double eps[MAXS][9];             // e_a
+  // the stress-spectrum form of the contraction (contract_point): s_b = C:e_b
+  // and w_t sym(e_a)_t / mu over the six symmetric components
+  // t = (00, 01, 02, 11, 12, 22), w_t = 1 on the diagonal and 2 off it
+  double sig6[MAXS][6];
+  double e6m[MAXS][6];
+  float sig6f[MAXS][6];
+  float e6mf[MAXS][6];
   double d[3];                     // grid spacing
   const double* freq[3];           // per-axis frec tables (device), freq[a][i] = frec1(i, n[a], d[a])
   double g_iso;                    // (mu+ll) / (mu^2 kk) for mup == 0
@@ -526,17 +531,12 @@ __global__ void k_set_int(int* p, int v) {
 
 // Contraction at one k-point (G.c:4609-4629): out_a = sum_b B_ab(k) v_b,
 // written to out[a * ostride] (v stays in registers, so out may alias the
-// locations v was read from).  Stored mode reads B from Bsym.  Recompute
-// mode never forms B: with u_a = s_a f (s_a = C:e_a), w = sum_b u_b v_b and
-// G symmetric,
-//   sum_b B_ab v_b = sum_b (s_a:e_b / mu) v_b - u_a . (G w) / mu,
-// the same B_ab = (s_a:e_b - u_a.G.u_b)/mu of Bmatrix (G.c:386-402) with the
-// sums regrouped; u and G/mu are evaluated in FP64 and rounded to T, and the
-// sums run in T.  With FP32 and unrolled outputs the rounded u_a stay in
-// registers; otherwise they are recomputed for the second sum.  quirk != 0 adds v_a * sum_ij DD_a,ij
-// (virtualevolv G.c:1426-1441) in a second pass over a.  UNROLL_A unrolls
-// the loop over outputs (more registers); rolled, v is still only indexed by
-// unrolled loops, so it stays in registers.
+// locations v was read from).  Stored mode reads B from Bsym; recompute mode
+// never forms B and takes the stress-spectrum form (contract_stress_form).
+// quirk != 0 adds v_a * sum_ij DD_a,ij (virtualevolv G.c:1426-1441) in a
+// second pass over a.  UNROLL_A unrolls the stored mode's loop over outputs
+// (more registers); rolled, v is still only indexed by unrolled loops, so it
+// stays in registers.
 __device__ __forceinline__ void sig_f(const ElasticConsts* __restrict__ ec, int a, const double f[3], double u[3]) {
   const double* s = ec->sig[a];
   u[0] = s[0] * f[0] + s[1] * f[1] + s[2] * f[2];
@@ -544,39 +544,85 @@ __device__ __forceinline__ void sig_f(const ElasticConsts* __restrict__ ec, int 
   u[2] = s[6] * f[0] + s[7] * f[1] + s[8] * f[2];
 }
 
-template <typename T> __device__ __forceinline__ T se_mu(const ElasticConsts* __restrict__ ec, int a, int b);
-template <> __device__ __forceinline__ double se_mu<double>(const ElasticConsts* __restrict__ ec, int a, int b) {
-  return ec->SEm[a][b];
+template <typename T> __device__ __forceinline__ const T* sig6_of(const ElasticConsts* __restrict__ ec, int b);
+template <> __device__ __forceinline__ const double* sig6_of<double>(const ElasticConsts* __restrict__ ec, int b) {
+  return ec->sig6[b];
 }
-template <> __device__ __forceinline__ float se_mu<float>(const ElasticConsts* __restrict__ ec, int a, int b) {
-  return ec->SEmf[a][b];
+template <> __device__ __forceinline__ const float* sig6_of<float>(const ElasticConsts* __restrict__ ec, int b) {
+  return ec->sig6f[b];
+}
+template <typename T> __device__ __forceinline__ const T* e6m_of(const ElasticConsts* __restrict__ ec, int a);
+template <> __device__ __forceinline__ const double* e6m_of<double>(const ElasticConsts* __restrict__ ec, int a) {
+  return ec->e6m[a];
+}
+template <> __device__ __forceinline__ const float* e6m_of<float>(const ElasticConsts* __restrict__ ec, int a) {
+  return ec->e6mf[a];
 }
 
-template <typename T, int SM>
-__device__ __forceinline__ cx<T> contract_row(int a, const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec,
-                                              long pt, int S, bool stored, const T* __restrict__ Bsym, long P,
-                                              const double f[3], const T (&ua)[3], const cx<T> (&gw)[3]) {
-  cx<T> acc = {(T)0, (T)0};
-  if (stored) {
+// Recomputed B(k) in the stress-spectrum form.  With s = sum_b s_b v_b
+// (s_b = C:e_b symmetric, so s_a:e_b = e_a:s_b), t = s f and g = G t,
+//   sum_b B_ab v_b = (e_a : s - u_a . g) / mu = e_a : r / mu,
+//   r = s - C : sym(g (x) f),
+// the B_ab = (s_a:e_b - u_a.G.u_b)/mu of Bmatrix (G.c:386-402) with u_a =
+// s_a f and the sums regrouped: 6 S + 6 S complex-by-real products per
+// k-point instead of S^2 + 6 S.  For the cubic C of cijkl,
+// C:X = lambda tr(X) I + 2 mu X + mu' diag(X).  f and G are evaluated in
+// FP64 and rounded to T; the sums run in T.  Writes out[a * ostride] (v
+// stays in registers, so out may alias where v was read from).
+template <typename T, int SM, bool NTS>
+__device__ __forceinline__ void contract_stress_form(const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec,
+                                                     const double f[3], const double g[6], int S, cx<T>* out,
+                                                     long ostride) {
+  cx<T> sv[6];
 #pragma unroll
-    for (int b = 0; b < SM; b++) {
-      if (b >= S) continue;
-      const T bab = Bsym[(long)tri_index(a, b, S) * P + pt];
-      acc.x += v[b].x * bab;
-      acc.y += v[b].y * bab;
-    }
-  } else {
+  for (int t = 0; t < 6; t++) sv[t] = {(T)0, (T)0};
 #pragma unroll
-    for (int b = 0; b < SM; b++) {
-      if (b >= S) continue;
-      const T se = se_mu<T>(ec, a, b);
-      acc.x += v[b].x * se;
-      acc.y += v[b].y * se;
+  for (int b = 0; b < SM; b++) {
+    if (b >= S) continue;
+    const T* sb = sig6_of<T>(ec, b);
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      sv[t].x += sb[t] * v[b].x;
+      sv[t].y += sb[t] * v[b].y;
     }
-    acc.x -= ua[0] * gw[0].x + ua[1] * gw[1].x + ua[2] * gw[2].x;
-    acc.y -= ua[0] * gw[0].y + ua[1] * gw[1].y + ua[2] * gw[2].y;
   }
-  return acc;
+  const T f0 = (T)f[0], f1 = (T)f[1], f2 = (T)f[2];
+  // t = s f
+  cx<T> tv[3];
+  tv[0] = {sv[0].x * f0 + sv[1].x * f1 + sv[2].x * f2, sv[0].y * f0 + sv[1].y * f1 + sv[2].y * f2};
+  tv[1] = {sv[1].x * f0 + sv[3].x * f1 + sv[4].x * f2, sv[1].y * f0 + sv[3].y * f1 + sv[4].y * f2};
+  tv[2] = {sv[2].x * f0 + sv[4].x * f1 + sv[5].x * f2, sv[2].y * f0 + sv[4].y * f1 + sv[5].y * f2};
+  // g = G t
+  const T g0 = (T)g[0], g1 = (T)g[1], g2 = (T)g[2], g3 = (T)g[3], g4 = (T)g[4], g5 = (T)g[5];
+  cx<T> gv[3];
+  gv[0] = {g0 * tv[0].x + g1 * tv[1].x + g2 * tv[2].x, g0 * tv[0].y + g1 * tv[1].y + g2 * tv[2].y};
+  gv[1] = {g1 * tv[0].x + g3 * tv[1].x + g4 * tv[2].x, g1 * tv[0].y + g3 * tv[1].y + g4 * tv[2].y};
+  gv[2] = {g2 * tv[0].x + g4 * tv[1].x + g5 * tv[2].x, g2 * tv[0].y + g4 * tv[1].y + g5 * tv[2].y};
+  // r = s - C : Y, Y = sym(g (x) f): diagonal lambda tr(Y) + (2 mu + mu') Y_ii, off-diagonal mu (g_i f_j + g_j f_i)
+  const T lam = (T)ec->ll, mu = (T)ec->mu, dg = (T)(2 * ec->mu + ec->mup);
+  const cx<T> y00 = {gv[0].x * f0, gv[0].y * f0}, y11 = {gv[1].x * f1, gv[1].y * f1},
+              y22 = {gv[2].x * f2, gv[2].y * f2};
+  const cx<T> tr = {lam * (y00.x + y11.x + y22.x), lam * (y00.y + y11.y + y22.y)};
+  cx<T> rv[6];
+  rv[0] = {sv[0].x - (tr.x + dg * y00.x), sv[0].y - (tr.y + dg * y00.y)};
+  rv[3] = {sv[3].x - (tr.x + dg * y11.x), sv[3].y - (tr.y + dg * y11.y)};
+  rv[5] = {sv[5].x - (tr.x + dg * y22.x), sv[5].y - (tr.y + dg * y22.y)};
+  rv[1] = {sv[1].x - mu * (gv[0].x * f1 + gv[1].x * f0), sv[1].y - mu * (gv[0].y * f1 + gv[1].y * f0)};
+  rv[2] = {sv[2].x - mu * (gv[0].x * f2 + gv[2].x * f0), sv[2].y - mu * (gv[0].y * f2 + gv[2].y * f0)};
+  rv[4] = {sv[4].x - mu * (gv[1].x * f2 + gv[2].x * f1), sv[4].y - mu * (gv[1].y * f2 + gv[2].y * f1)};
+#pragma unroll
+  for (int a = 0; a < SM; a++) {
+    if (a >= S) continue;
+    const T* ea = e6m_of<T>(ec, a);
+    cx<T> o = {(T)0, (T)0};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      o.x += ea[t] * rv[t].x;
+      o.y += ea[t] * rv[t].y;
+    }
+    if constexpr (NTS) gstore<T>(out + a * ostride, o);
+    else out[a * ostride] = o;
+  }
 }
 
 template <typename T, int SM, bool UNROLL_A, bool NTS = false>
@@ -587,8 +633,6 @@ __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const Elast
     if constexpr (NTS) gstore<T>(q, x);
     else *q = x;
   };
-  // rounded u_a kept in registers (FP32, outputs unrolled) or recomputed
-  constexpr bool KEEP_U = sizeof(T) == 4 && UNROLL_A;
   double f[3], g[6];
   kvec(pt, ec, f);
   if (!green6(f, ec, g)) {  // B(0) = 0 and DD(0) = 0
@@ -597,59 +641,28 @@ __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const Elast
       if (a < S) put(out + a * ostride, {(T)0, (T)0});
     return;
   }
-  T uk[KEEP_U ? SM : 1][3];
-  cx<T> gw[3] = {{(T)0, (T)0}, {(T)0, (T)0}, {(T)0, (T)0}};
-  if (!stored) {
-    cx<T> w[3] = {{(T)0, (T)0}, {(T)0, (T)0}, {(T)0, (T)0}};
+  if (stored) {
+    auto row = [&](int a) -> cx<T> {
+      cx<T> acc = {(T)0, (T)0};
 #pragma unroll
-    for (int b = 0; b < SM; b++) {
-      if (b >= S) continue;
-      double ud[3];
-      sig_f(ec, b, f, ud);
-      const T u0 = (T)ud[0], u1 = (T)ud[1], u2 = (T)ud[2];
-      if constexpr (KEEP_U) {
-        uk[b][0] = u0;
-        uk[b][1] = u1;
-        uk[b][2] = u2;
+      for (int b = 0; b < SM; b++) {
+        if (b >= S) continue;
+        const T bab = Bsym[(long)tri_index(a, b, S) * P + pt];
+        acc.x += v[b].x * bab;
+        acc.y += v[b].y * bab;
       }
-      w[0].x += u0 * v[b].x; w[0].y += u0 * v[b].y;
-      w[1].x += u1 * v[b].x; w[1].y += u1 * v[b].y;
-      w[2].x += u2 * v[b].x; w[2].y += u2 * v[b].y;
-    }
-    // gw = (G / mu) w
-    const T g0 = (T)(g[0] * ec->inv_mu), g1 = (T)(g[1] * ec->inv_mu), g2 = (T)(g[2] * ec->inv_mu);
-    const T g3 = (T)(g[3] * ec->inv_mu), g4 = (T)(g[4] * ec->inv_mu), g5 = (T)(g[5] * ec->inv_mu);
-    gw[0].x = g0 * w[0].x + g1 * w[1].x + g2 * w[2].x;
-    gw[0].y = g0 * w[0].y + g1 * w[1].y + g2 * w[2].y;
-    gw[1].x = g1 * w[0].x + g3 * w[1].x + g4 * w[2].x;
-    gw[1].y = g1 * w[0].y + g3 * w[1].y + g4 * w[2].y;
-    gw[2].x = g2 * w[0].x + g4 * w[1].x + g5 * w[2].x;
-    gw[2].y = g2 * w[0].y + g4 * w[1].y + g5 * w[2].y;
-  }
-  auto row = [&](int a) -> cx<T> {
-    T ua[3] = {(T)0, (T)0, (T)0};
-    if (!stored) {
-      if constexpr (KEEP_U) {
-        ua[0] = uk[a][0];
-        ua[1] = uk[a][1];
-        ua[2] = uk[a][2];
-      } else {
-        double ud[3];
-        sig_f(ec, a, f, ud);
-        ua[0] = (T)ud[0];
-        ua[1] = (T)ud[1];
-        ua[2] = (T)ud[2];
-      }
-    }
-    return contract_row<T, SM>(a, v, ec, pt, S, stored, Bsym, P, f, ua, gw);
-  };
-  if (UNROLL_A) {
+      return acc;
+    };
+    if (UNROLL_A) {
 #pragma unroll
-    for (int a = 0; a < SM; a++)
-      if (a < S) put(out + a * ostride, row(a));
-  } else {
+      for (int a = 0; a < SM; a++)
+        if (a < S) put(out + a * ostride, row(a));
+    } else {
 #pragma unroll 1
-    for (int a = 0; a < S; a++) put(out + a * ostride, row(a));
+      for (int a = 0; a < S; a++) put(out + a * ostride, row(a));
+    }
+  } else {
+    contract_stress_form<T, SM, NTS>(v, ec, f, g, S, out, ostride);
   }
   if (quirk) {
     double tf[3];
@@ -961,6 +974,69 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
   __syncthreads();
   fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, tw);
   il_stage<T, EPT, false, false>(wb, sm, chunk, S, tile, goff, lidx);
+}
+
+// Column kernel, interleaved layout, with S, the tile (TC columns) and the
+// block (NT threads) fixed at compile time: a thread's loads of the tile are
+// all issued before its LDS stores, and its LDS reads before its stores to W,
+// so the whole tile is in flight at once; the k-points are contracted in the
+// stress-spectrum form; one k-point per thread.  Same arithmetic per k-point
+// and per pencil as k_col_contract_il.  grid = (column tiles, instances).
+template <typename T, int N, int S, int TC> struct ColIlFixed {
+  static constexpr int TPP = make_plan(N).tpp, LD = pitch<N>(), NPEN = S * TC, ROW = S * TC;
+  static constexpr int NT = N * TC;                // one k-point per thread
+  static constexpr int EL = N * ROW / NT;          // tile elements per thread (= S)
+  static constexpr int PPI = NT / TPP;             // pencils transformed at once
+  static constexpr int G = (NPEN + PPI - 1) / PPI; // pencils per TPP-thread group
+  static constexpr size_t LDS = (size_t)NPEN * LD * sizeof(cx<T>);
+};
+template <typename T, int N, int S, int TC>
+__global__ void __launch_bounds__((ColIlFixed<T, N, S, TC>::NT))
+k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int quirk,
+               const cx<T>* __restrict__ tw) {
+  using K = ColIlFixed<T, N, S, TC>;
+  constexpr int NT = K::NT, LD = K::LD, ROW = K::ROW, EL = K::EL, TPP = K::TPP;
+  static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024, "tile shape");
+  __shared__ cx<T> sm[K::NPEN * LD];
+  const long M = P / N;  // (i2, i3) columns
+  const long m0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
+  cx<T>* wb = W + ((long)blockIdx.y * P + m0) * S;  // row n of the tile at wb + n * M * S
+  const long rs = M * S;
+  const int t = threadIdx.x;
+  cx<T> buf[EL];
+#pragma unroll
+  for (int k = 0; k < EL; k++) {
+    const int e = t + k * NT, n = e / ROW, r = e % ROW;
+    buf[k] = wb[n * rs + r];
+  }
+#pragma unroll
+  for (int k = 0; k < EL; k++) {
+    const int e = t + k * NT, n = e / ROW, r = e % ROW;
+    sm[((r % S) * TC + r / S) * LD + n] = buf[k];  // pencil a * TC + c of element r = c * S + a
+  }
+  __syncthreads();
+  const int p0 = t / TPP, q = t % TPP;
+  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)-1, tw);
+  {
+    const int c = t / N, n = t % N;  // consecutive threads read consecutive LDS words
+    cx<T> v[S];
+#pragma unroll
+    for (int b = 0; b < S; b++) v[b] = sm[(b * TC + c) * LD + n];
+    contract_point<T, S, false>(v, ec, (long)n * M + m0 + c, S, quirk, false, (const T*)nullptr, P,
+                                sm + c * LD + n, (long)TC * LD);
+  }
+  __syncthreads();
+  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)1, tw);
+#pragma unroll
+  for (int k = 0; k < EL; k++) {
+    const int e = t + k * NT, n = e / ROW, r = e % ROW;
+    buf[k] = sm[((r % S) * TC + r / S) * LD + n];
+  }
+#pragma unroll
+  for (int k = 0; k < EL; k++) {
+    const int e = t + k * NT, n = e / ROW, r = e % ROW;
+    gstore<T>(wb + n * rs + r, buf[k]);
+  }
 }
 
 template <typename T, int N> constexpr size_t col_il_lds_bytes(int S, int tile) {
