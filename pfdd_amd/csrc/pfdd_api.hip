@@ -86,6 +86,7 @@ struct pfdd_ctx {
   int plane = 0;             // 1: the LDS-resident plane kernel does the N2/N3 passes and the update
   int il = 0;                // 1: W interleaved, [B][P][S] (fields of a point together); else [B][S][P]
   int il_fixed = 0;          // 1: the interleaved column kernel with compile-time S and tile (k_col_il_fixed)
+  int il_fixed_row = 0;      // 1: the interleaved row kernel with compile-time S and rows (k_row_il_fixed)
   size_t c3_lds = 0;         // its dynamic LDS bytes
   int plan = 0;              // the applied PFDD_PLAN_* word
   int r4 = 0;                // bit ax: radix-4 sequence for the standalone passes along axis ax
@@ -369,6 +370,24 @@ int k1il_launch_n(pfdd_ctx* c, bool attr_only) {
   return rc;
 }
 
+// Rows per block of the compile-time interleaved row kernel (k_row_il_fixed)
+// for ns fields along n3, or 0 where it is not instantiated.
+int il_fixed_rows(int S, int n3) { return S == 12 && (n3 == 128 || n3 == 256) ? 1 : 0; }
+template <typename T, int N>
+bool k1il_fixed(pfdd_ctx* c) {
+  if constexpr (N == 128 || N == 256) {
+    if (!c->il_fixed_row || c->mask || c->taupp) return false;
+    with_master<T>(c, [&](auto xi) {
+      using M = decltype(xi);
+      hipLaunchKernelGGL((k_row_il_fixed<T, N, 12, 1, M>), dim3((unsigned)c->row_nblk, (unsigned)c->B),
+                         dim3(RowIlFixed<T, N, 12, 1>::NT), 0, c->stream, (cx<T>*)c->W, xi, c->sa,
+                         tw_of<T>(c->tw[2]), c->partials, c->step_ctr);
+    });
+    return true;
+  }
+  return false;
+}
+
 template <typename T>
 int k1il_attr(pfdd_ctx* c) {
   switch (c->p.n3) {
@@ -406,12 +425,13 @@ void finish_rows(pfdd_ctx* c, int* step_ctr) {
 template <typename T>
 int k1_launch(pfdd_ctx* c, bool finish = true) {
   switch (c->p.n3) {
-#define PFDD_L(NN)                    \
-  case NN:                            \
-    if (c->il)                        \
-      k1il_launch_n<T, NN>(c, false); \
-    else                              \
-      k1_launch_n<T, NN>(c);          \
+#define PFDD_L(NN)                                        \
+  case NN:                                                \
+    if (c->il) {                                          \
+      if (!k1il_fixed<T, NN>(c)) k1il_launch_n<T, NN>(c, false); \
+    } else {                                              \
+      k1_launch_n<T, NN>(c);                              \
+    }                                                     \
     break;
     PFDD_L(32) PFDD_L(48) PFDD_L(64) PFDD_L(96) PFDD_L(128) PFDD_L(192) PFDD_L(256) PFDD_L(384) PFDD_L(512)
 #undef PFDD_L
@@ -959,6 +979,7 @@ int apply_plan(pfdd_ctx* c, int plan) {
   c->plane = plane;
   c->il = il;
   c->il_fixed = il && c3_tile > 0 && c3_tile == il_fixed_tile(S, p.n1, c->fp64 != 0) && getenv("PFDD_IL_FIXED");
+  c->il_fixed_row = il && row_tile == il_fixed_rows(S, p.n3) && getenv("PFDD_IL_FIXED");
   int rc = PFDD_OK;
   if (il && (rc = c->fp64 ? k1il_attr<double>(c) : k1il_attr<float>(c))) return rc;
   if (c3_tile > 0 && (rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true))) return rc;

@@ -932,6 +932,95 @@ k_row_step_il(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
   il_stage<T, EPT, false, BATCH>(wb, sm, chunk, S, N, goff, lidx);
 }
 
+// Row kernel, interleaved layout, with S and the rows per block (R) fixed at
+// compile time, no frozen points and a uniform shear: one grid point per
+// thread, whose S master values are loaded before the inverse transform so
+// that they arrive while it runs, and whose tile elements are loaded (and
+// later stored) in one batch.  Per point and per pencil the same arithmetic
+// as k_row_step_il; each field's block sum adds the threads' values in
+// thread order.  grid = (row blocks, instances).
+template <typename T, int N, int S, int R> struct RowIlFixed {
+  static constexpr int TPP = make_plan(N).tpp, LD = pitch<N>(), NPEN = S * R;
+  static constexpr int NT = N * R;                  // one grid point per thread
+  static constexpr int PPI = NT / TPP;              // pencils transformed at once
+  static constexpr int G = (NPEN + PPI - 1) / PPI;  // pencils per TPP-thread group
+  static constexpr size_t LDS = (size_t)NPEN * LD * sizeof(cx<T>);
+};
+template <typename T, int N, int S, int R, typename M>
+__global__ void __launch_bounds__((RowIlFixed<T, N, S, R>::NT))
+k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict__ tw,
+               double* __restrict__ partials, const int* __restrict__ step_ctr) {
+  using K = RowIlFixed<T, N, S, R>;
+  constexpr int NT = K::NT, LD = K::LD, TPP = K::TPP, NW = NT / 64;
+  static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024, "tile shape");
+  __shared__ cx<T> sm[K::NPEN * LD];
+  __shared__ double red[S][NW];
+  const long P = a.P;
+  const long nblk = P / ((long)N * R);
+  const long blk = blockIdx.x;
+  const int inst = blockIdx.y, t = threadIdx.x;
+  const long pt0 = blk * R * N;
+  cx<T>* wb = W + ((long)inst * P + pt0) * S;  // the block's R*N*S contiguous elements
+  // the thread's point: row r of the block, position i along N3
+  const int r = t / N, i = t % N;
+  double2 xv[S];
+#pragma unroll
+  for (int f = 0; f < S; f++) xv[f] = xi.load(((long)inst * S + f) * P + pt0 + t);
+  {
+    cx<T> buf[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) buf[k] = wb[t + k * NT];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      const int e = t + k * NT, po = e / S, f = e % S;  // element e: point po, field f
+      sm[(f * R + po / N) * LD + po % N] = buf[k];     // pencil f * R + row
+    }
+  }
+  __syncthreads();
+  const int p0 = t / TPP, q = t % TPP;
+  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)1, tw);
+  const double inv_n = 1.0 / (double)P;
+  double acc[S];
+#pragma unroll
+  for (int f = 0; f < S; f++) {
+    const FieldCoef c = field_coef(a, inst * S + f, step_ctr);
+    cx<T>* cell = sm + (f * R + r) * LD + i;
+    const cx<T> d = *cell;
+    double2 x = xv[f];
+    gl_point<false, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, nullptr, pt0 + t, inv_n);
+    xi.store(((long)inst * S + f) * P + pt0 + t, x);
+    *cell = {(T)x.x, (T)x.y};
+    acc[f] = x.x;  // (scaled by 1/P once per field)
+  }
+  __syncthreads();
+  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)-1, tw);
+  {
+    cx<T> buf[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      const int e = t + k * NT, po = e / S, f = e % S;
+      buf[k] = sm[(f * R + po / N) * LD + po % N];
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) gstore<T>(wb + t + k * NT, buf[k]);
+  }
+  // per-field block sums: each wave's lanes in a fixed butterfly, then the waves in order
+#pragma unroll
+  for (int f = 0; f < S; f++) {
+    double v = acc[f];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((t & 63) == 0) red[f][t >> 6] = v;
+  }
+  __syncthreads();
+  if (t < S) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) sum += red[t][w];
+    partials[((long)inst * S + t) * nblk + blk] = sum * inv_n;
+  }
+}
+
 template <typename T, int N> constexpr size_t row_il_lds_bytes(int S, int R, int nt) {
   return (size_t)S * R * pitch<N>() * sizeof(cx<T>) + (size_t)nt * sizeof(double);
 }
