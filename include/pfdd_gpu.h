@@ -36,7 +36,8 @@ enum {
   PFDD_E_RADIX = -4,        /* a grid edge is not a supported FFT length */
   PFDD_E_NAN = -5,          /* non-finite value in xi or in a parameter */
   PFDD_E_UNSUPPORTED = -6,  /* input outside what the GL path supports */
-  PFDD_E_STATE = -7         /* call order (e.g. steps before upload) */
+  PFDD_E_STATE = -7,        /* call order (e.g. steps before upload) */
+  PFDD_E_LIMIT = -8         /* input larger than the fixed staging array of pfdd_fft_host_noalloc */
 };
 
 /* Precision of FFT + contraction.  PFDD_MIXED: FP32 FFTs and contraction,
@@ -354,6 +355,21 @@ void fft_forward(float *data, int batch);
 void fft_backward(float *data, int batch);
 int pfdd_fft_host_status(void);
 void pfdd_fft_host_release(void);
+
+/* Allocation-free form of the host seam.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  fft_forward /
+ * fft_backward stage the batch in a device buffer that they allocate and
+ * grow; this call stages it in a fixed device array of
+ * PFDD_NOALLOC_MAX_POINTS complex floats (16 MiB), which each GPU holds from
+ * its first pfdd_fft_host_init until the process ends, and it calls no
+ * allocating routine.  dir = PFDD_FORWARD or PFDD_BACKWARD; the result is
+ * the same as that of fft_forward / fft_backward.  Returns 0, PFDD_E_LIMIT
+ * when batch * N1*N2*N3 > PFDD_NOALLOC_MAX_POINTS, PFDD_E_STATE before
+ * pfdd_fft_host_init, or another PFDD_E_* code.  Synchronous.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define PFDD_NOALLOC_MAX_POINTS (1 << 21)
+int pfdd_fft_host_noalloc(float *data, int batch, int dir);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libpfdd_gpu.so")
 
 MAX_NS = 16
-OK, E_HIP, E_ALLOC, E_SHAPE, E_RADIX, E_NAN, E_UNSUPPORTED, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7
+OK, E_HIP, E_ALLOC, E_SHAPE, E_RADIX, E_NAN, E_UNSUPPORTED, E_STATE, E_LIMIT = 0, -1, -2, -3, -4, -5, -6, -7, -8
+NOALLOC_MAX_POINTS = 1 << 21
 FP64, MIXED, MIXED_LEAN = 0, 1, 2
 B_STORED, B_RECOMPUTE = 0, 1
 FORWARD, BACKWARD = -1, 1
@@ -58,7 +59,7 @@ EXPORTED = [
     "pfdd_ensemble_gl_steps", "pfdd_ensemble_autotune", "pfdd_ensemble_set_plan", "pfdd_ensemble_sync",
     "pfdd_ensemble_stream", "pfdd_ensemble_device_bytes", "pfdd_time_step_kernels", "pfdd_ensemble_set_mode",
     "pfdd_ensemble_get_mode", "pfdd_set_line_points", "pfdd_line_sums", "pfdd_gl_steps_lines", "pfdd_line_points",
-    "pfdd_plastic_converge",
+    "pfdd_plastic_converge", "pfdd_fft_host_noalloc",
 ]
 ENS_PHASED, ENS_PERSISTENT = 0, 1
 (STAGE_UPDATE, STAGE_FFT_AXIS3, STAGE_FFT_AXIS2, STAGE_FFT_AXIS1, STAGE_CONTRACT, STAGE_STEP, STAGE_COPY,
@@ -142,6 +143,7 @@ def load():
     L.pfdd_fft_host_status.argtypes = []
     L.pfdd_fft_host_release.argtypes = []
     L.pfdd_fft_host_release.restype = None
+    L.pfdd_fft_host_noalloc.argtypes = [vp, ci, ci]
     fp = ctypes.POINTER(ctypes.c_float)
     L.pfdd_time_step_kernels.argtypes = [vp, ci, vp, vp]
     L.pfdd_set_line_points.argtypes = [vp, vp, ci]

@@ -823,6 +823,7 @@ const char* pfdd_strerror(int rc) {
     case PFDD_E_NAN: return "non-finite value";
     case PFDD_E_UNSUPPORTED: return "unsupported input";
     case PFDD_E_STATE: return "call out of order";
+    case PFDD_E_LIMIT: return "exceeds the fixed staging array of an allocation-free call";
     default: return "unknown error";
   }
 }
@@ -2308,8 +2309,19 @@ struct HostSeam {
   float2* buf = nullptr;
   size_t cap = 0;  // complex elements
   int status = PFDD_E_STATE;
+  int dev = -1;    // device of ctx (current at pfdd_fft_host_init)
 };
 HostSeam g_seam;
+
+// Staging buffer of the allocation-free seam call (pfdd_fft_host_noalloc):
+// a fixed device array, one per GPU for the life of the process.  In host
+// code `scratch` is only a symbol; its device address on each GPU comes from
+// hipGetSymbolAddress (taken once per device in pfdd_fft_host_init, with that
+// device current) and is the only form passed to copies and kernels.
+constexpr size_t NOALLOC_BYTES = (size_t)PFDD_NOALLOC_MAX_POINTS * sizeof(float2);
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
 
 int seam_run(float* data, int batch, int dir) {
   if (!g_seam.ctx) return PFDD_E_STATE;
@@ -2360,8 +2372,35 @@ int pfdd_fft_host_init(int n1, int n2, int n3) {
   p.dslip[0] = 1.0;
   p.uq2 = 1.0;
   p.pi_const = 3.141592654;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return g_seam.status = PFDD_E_HIP;
+  // the allocation-free call's staging array on this device: its address, once per device
+  if (dev >= 0 && dev < NOALLOC_MAX_DEVICES && !g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return g_seam.status = PFDD_E_HIP;
+    g_scratch_ptr[dev] = ptr;
+  }
   g_seam.status = pfdd_create(&p, &g_seam.ctx);
+  g_seam.dev = g_seam.status == PFDD_OK ? dev : -1;
   return g_seam.status;
+}
+
+int pfdd_fft_host_noalloc(float* data, int batch, int dir) {
+  if (!g_seam.ctx || g_seam.dev < 0 || g_seam.dev >= NOALLOC_MAX_DEVICES || !g_scratch_ptr[g_seam.dev])
+    return PFDD_E_STATE;
+  if (!data || batch < 1) return PFDD_E_SHAPE;
+  if (dir != PFDD_FORWARD && dir != PFDD_BACKWARD) return PFDD_E_SHAPE;
+  pfdd_ctx* c = g_seam.ctx;
+  if ((long long)batch * c->P > (long long)PFDD_NOALLOC_MAX_POINTS) return PFDD_E_LIMIT;
+  const size_t n = (size_t)batch * c->P;
+  // every element the transform reads is copied in first: nothing has to start at zero
+  float2* buf = (float2*)g_scratch_ptr[g_seam.dev];
+  HIPCK(hipMemcpyAsync(buf, data, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+  const int rc = fft3d<float>(c, buf, batch, dir);
+  if (rc) return rc;
+  HIPCK(hipMemcpyAsync(data, buf, n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return PFDD_OK;
 }
 
 void fft_forward(float* data, int batch) { g_seam.status = seam_run(data, batch, PFDD_FORWARD); }

@@ -57,7 +57,8 @@ def test_header_constants_match_package():
 
 
 def test_strerror_names_codes():
-    for rc in (pa.OK, pa.E_HIP, pa.E_ALLOC, pa.E_SHAPE, pa.E_RADIX, pa.E_NAN, pa.E_UNSUPPORTED, pa.E_STATE):
+    for rc in (pa.OK, pa.E_HIP, pa.E_ALLOC, pa.E_SHAPE, pa.E_RADIX, pa.E_NAN, pa.E_UNSUPPORTED, pa.E_STATE,
+               pa.E_LIMIT):
         assert pa.strerror(rc) and pa.strerror(rc) != "unknown error"
 
 
@@ -212,6 +213,7 @@ def test_fft_seam_requires_init():
     a = np.zeros(8, dtype=np.complex64)
     L.fft_forward(a.ctypes.data, 1)
     assert L.pfdd_fft_host_status() == pa.E_STATE
+    assert L.pfdd_fft_host_noalloc(a.ctypes.data, 1, pa.FORWARD) == pa.E_STATE
 
 
 def _captured_stdout(fn):
