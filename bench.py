@@ -48,12 +48,27 @@ def stage_bytes(stage, S, P, ntri, fp64, stored, lean=False):
     raise ValueError(stage)
 
 
+def fixed_interleaved(shape, S, stored):
+    """AUTO runs the interleaved layout with the compile-time row and column
+    kernels (k_row_il_fixed, k_col_il_fixed) for S = 12 at N1 = N3 = 128 with
+    recomputed B (apply_plan in pfdd_api.hip)."""
+    return S == 12 and shape[0] == 128 and shape[2] == 128 and not stored
+
+
 def rocprof_name(stage, plan, fp64, shape, S, stored, lean=False):
-    """The kernel symbol rocprofv3 reports for a stage of this context."""
+    """The kernel symbol rocprofv3 reports for a stage of this context (AUTO plan)."""
     T = "double" if fp64 else "float"
     M = "pfdd::Master<double> " if fp64 else "pfdd::MasterF32L" if lean else "pfdd::MasterF32C"
     n1, n2, n3 = shape
     sm = 4 if S <= 4 else 8 if S <= 8 else 12 if S <= 12 else 16
+    if fixed_interleaved(shape, S, stored):
+        if stage == pa.STAGE_UPDATE:
+            return f"void pfdd::k_row_il_fixed<{T}, {n3}, 12, 1, {M}>"
+        if stage == pa.STAGE_CONTRACT:
+            return f"void pfdd::k_col_il_fixed<{T}, {n1}, 12, {1 if fp64 else 2}>"
+        if stage == pa.STAGE_FFT_AXIS2:
+            return f"void pfdd::k_fft_cols<{T}, {n2}, 8, false>"
+        return None
     if stage == pa.STAGE_UPDATE:
         return (f"void pfdd::k_row_step<{T}, {n3}, false, false, {M}>" if plan[pa.STAGE_FFT_AXIS3] == 0
                 else f"void pfdd::k_update<{T}, false, false>")
@@ -122,10 +137,11 @@ def time_kernels(ctx, nsteps=20):
     plan = ctx.step_plan()
     names = {st: n for n, st, _ in STAGES}
     names[pa.STAGE_FINISH] = "k_finish_sums"
+    fixed = fixed_interleaved(ctx.shape, ctx.ns, ctx.params.bmode == pa.B_STORED)
     if plan[pa.STAGE_FFT_AXIS3] == 0:
-        names[pa.STAGE_UPDATE] = "k_row_step(N3 inv + update + N3 fwd)"
+        names[pa.STAGE_UPDATE] = ("k_row_il_fixed" if fixed else "k_row_step") + "(N3 inv + update + N3 fwd)"
     if plan[pa.STAGE_FFT_AXIS1] == 0:
-        names[pa.STAGE_CONTRACT] = "k_col_contract(N1 fwd + B(k) + N1 inv)"
+        names[pa.STAGE_CONTRACT] = ("k_col_il_fixed" if fixed else "k_col_contract") + "(N1 fwd + B(k) + N1 inv)"
     out = {}
     for st, (ms, n) in ctx.time_step_kernels(nsteps).items():
         out[names[st]] = (st, n, ms / n)

@@ -372,10 +372,10 @@ int k1il_launch_n(pfdd_ctx* c, bool attr_only) {
 
 // Rows per block of the compile-time interleaved row kernel (k_row_il_fixed)
 // for ns fields along n3, or 0 where it is not instantiated.
-int il_fixed_rows(int S, int n3) { return S == 12 && (n3 == 128 || n3 == 256) ? 1 : 0; }
+int il_fixed_rows(int S, int n3) { return S == 12 && n3 == 128 ? 1 : 0; }
 template <typename T, int N>
 bool k1il_fixed(pfdd_ctx* c) {
-  if constexpr (N == 128 || N == 256) {
+  if constexpr (N == 128) {
     if (!c->il_fixed_row || c->mask || c->taupp) return false;
     with_master<T>(c, [&](auto xi) {
       using M = decltype(xi);
@@ -468,30 +468,27 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
 // Tile of the compile-time interleaved column kernel (k_col_il_fixed) for
 // ns fields along n1, or 0 where it is not instantiated.
 int il_fixed_tile(int S, int n1, bool fp64) {
-  if (S != 12) return 0;
-  if (n1 == 128) return fp64 ? 2 : 4;
-  if (n1 == 256) return fp64 ? 1 : 2;
-  return 0;
+  if (S != 12 || n1 != 128) return 0;
+  return fp64 ? 1 : 2;
 }
-template <typename T, int N, int TC>
-void k3il_fixed(pfdd_ctx* c, int quirk) {
+template <typename T, int N>
+void k3il_fixed(pfdd_ctx* c) {
+  constexpr int TC = sizeof(T) == 8 ? 1 : 2;
   const long nb = ((long)c->p.n2 * c->p.n3) / TC;
   hipLaunchKernelGGL((k_col_il_fixed<T, N, 12, TC>), dim3((unsigned)nb, (unsigned)c->B),
-                     dim3(ColIlFixed<T, N, 12, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P, quirk,
+                     dim3(ColIlFixed<T, N, 12, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P,
                      tw_of<T>(c->tw[0]));
 }
 
 // Column kernel, interleaved layout (c->c3_tile columns of every field per block).
 template <typename T, int N>
 int k3il_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
-  if (!attr_only && c->il_fixed) {
-    if constexpr (N == 128) {
-      k3il_fixed<T, 128, sizeof(T) == 8 ? 2 : 4>(c, quirk);
-    } else if constexpr (N == 256) {
-      k3il_fixed<T, 256, sizeof(T) == 8 ? 1 : 2>(c, quirk);
+  if constexpr (N == 128) {
+    if (!attr_only && c->il_fixed && !quirk) {  // (the prologue's virtualevolv term takes the generic kernel)
+      k3il_fixed<T, N>(c);
+      tap(c, PFDD_STAGE_CONTRACT);
+      return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
     }
-    tap(c, PFDD_STAGE_CONTRACT);
-    return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
   }
   const long nb = ((long)c->p.n2 * c->p.n3) / c->c3_tile;
   const int nt = il_threads<N>(c->S * c->c3_tile);
@@ -882,7 +879,7 @@ int il_tiles(pfdd_ctx* c, int plan, int& row_tile, int& c3_tile, size_t& c3_lds)
     if (creq) {
       t = 1 << (creq - 1);
       if (!col_fits(t)) return PFDD_E_SHAPE;
-    } else if (const int ft = il_fixed_tile(S, p.n1, c->fp64 != 0); ft > 0 && getenv("PFDD_IL_FIXED")) {
+    } else if (const int ft = il_fixed_tile(S, p.n1, c->fp64 != 0); ft > 0) {
       t = ft;
     } else {
       while (col_fits(t * 2) && col_lds(t * 2) <= IL_AUTO_LDS && (size_t)t * 2 * S * csz <= 256) t *= 2;
@@ -919,7 +916,14 @@ int apply_plan(pfdd_ctx* c, int plan) {
   int row_tile = 0, row_nblk = 0, c3_tile = 0, plane = 0, il = 0;
   size_t c3_lds = 0;
   if ((plan & PFDD_PLAN_INTERLEAVED) && (plan & PFDD_PLAN_FIELD_MAJOR)) return PFDD_E_SHAPE;
-  if (plan & PFDD_PLAN_INTERLEAVED) {
+  // AUTO takes the interleaved layout where its compile-time row and column
+  // kernels exist (S = 12 at N1 = N3 = 128, recomputed B): 0.749 against
+  // 0.918 ms per FP64 step and 0.461 against 0.490 ms per mixed step at
+  // 128^3 x 12.  Without them the field-major plans measured faster.
+  const bool auto_il = !(plan & (PFDD_PLAN_FIELD_MAJOR | PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS |
+                                 PFDD_PLAN_FUSED_COLS | PFDD_PLAN_PLANE | PFDD_PLAN_NO_PLANE)) &&
+                       !c->stored && il_fixed_tile(S, p.n1, c->fp64 != 0) > 0 && il_fixed_rows(S, p.n3) > 0;
+  if ((plan & PFDD_PLAN_INTERLEAVED) || auto_il) {
     // interleaved layout: the interleaved row kernel, no plane kernel
     if (!fused_len(p.n3) || (plan & (PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_PLANE))) return PFDD_E_SHAPE;
     const int rc = il_tiles(c, plan, row_tile, c3_tile, c3_lds);
@@ -979,8 +983,8 @@ int apply_plan(pfdd_ctx* c, int plan) {
   c->c3_lds = c3_lds;
   c->plane = plane;
   c->il = il;
-  c->il_fixed = il && c3_tile > 0 && c3_tile == il_fixed_tile(S, p.n1, c->fp64 != 0) && getenv("PFDD_IL_FIXED");
-  c->il_fixed_row = il && row_tile == il_fixed_rows(S, p.n3) && getenv("PFDD_IL_FIXED");
+  c->il_fixed = il && c3_tile > 0 && c3_tile == il_fixed_tile(S, p.n1, c->fp64 != 0);
+  c->il_fixed_row = il && row_tile == il_fixed_rows(S, p.n3);
   int rc = PFDD_OK;
   if (il && (rc = c->fp64 ? k1il_attr<double>(c) : k1il_attr<float>(c))) return rc;
   if (c3_tile > 0 && (rc = c->fp64 ? k3_launch<double>(c, 0, true) : k3_launch<float>(c, 0, true))) return rc;

@@ -165,12 +165,18 @@ template <typename T> struct Bfly<T, 8> {
 // the first Stockham pass can load straight from HBM into registers and the
 // last pass can store straight back; only the passes in between go through
 // LDS.  Element i of pencil p:
-template <typename T> struct LdsPencils {     // sm[p * ld + i]
+// Position i of a pencil sits at i + (i >> SH) when SH > 0: one pad slot per
+// 2^SH points, so that the first radix-8 pass's writes (stride 8 within a
+// pencil) fall on distinct LDS banks.
+template <int SH> __device__ __forceinline__ int lds_pos(int i) { return SH ? i + (i >> SH) : i; }
+template <int N, int SH> constexpr int padded_pitch() { return SH ? N + (N >> SH) + 1 : N + 1; }
+
+template <typename T, int SH = 0> struct LdsPencils {     // sm[p * ld + lds_pos<SH>(i)]
   cx<T>* sm;
   int ld;
   static constexpr bool in_lds = true;
-  __device__ __forceinline__ cx<T> load(int p, int i) const { return sm[p * ld + i]; }
-  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { sm[p * ld + i] = v; }
+  __device__ __forceinline__ cx<T> load(int p, int i) const { return sm[p * ld + lds_pos<SH>(i)]; }
+  __device__ __forceinline__ void store(int p, int i, cx<T> v) const { sm[p * ld + lds_pos<SH>(i)] = v; }
 };
 template <typename T> struct RowPencils {      // contiguous rows: base[p * n + i]
   cx<T>* base;
@@ -280,11 +286,12 @@ __device__ __forceinline__ void fft_pencils(const In& in, const Mid& lds, const 
   fft_from<T, N, G, 0, RL>(in, lds, out, p0, pstride, npen, q, s, tw);
 }
 
-// All-LDS transforms of up to G pencils per thread (pencil p at sm + p*LD).
-template <typename T, int N, int G, int LD>
+// All-LDS transforms of up to G pencils per thread (pencil p at sm + p*LD,
+// position i at lds_pos<SH>(i)).
+template <typename T, int N, int G, int LD, int SH = 0>
 __device__ __forceinline__ void fft_lds_multi(cx<T>* sm, int p0, int pstride, int npen, int q, T s,
                                               const cx<T>* __restrict__ tw) {
-  const LdsPencils<T> l{sm, LD};
+  const LdsPencils<T, SH> l{sm, LD};
   fft_pencils<T, N, G>(l, l, l, p0, pstride, npen, q, s, tw);
 }
 

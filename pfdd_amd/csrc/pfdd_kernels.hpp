@@ -182,12 +182,12 @@ __device__ __forceinline__ void gmul(const double g[6], const double u[3], doubl
   o[2] = g[2] * u[0] + g[4] * u[1] + g[5] * u[2];
 }
 
+// k-vector of flat point index pt (< 2^31: every edge is at most 1024)
 __device__ __forceinline__ void kvec(long pt, const ElasticConsts* __restrict__ ec, double f[3]) {
-  const int n2 = ec->n[1], n3 = ec->n[2];
-  const int k = (int)(pt % n3);
-  const long r = pt / n3;
-  const int j = (int)(r % n2);
-  const int i = (int)(r / n2);
+  const unsigned n2 = (unsigned)ec->n[1], n3 = (unsigned)ec->n[2];
+  const unsigned p = (unsigned)pt;
+  const unsigned r = p / n3, k = p - r * n3;
+  const unsigned i = r / n2, j = r - i * n2;
   f[0] = ec->freq[0][i];
   f[1] = ec->freq[1][j];
   f[2] = ec->freq[2][k];
@@ -625,6 +625,67 @@ __device__ __forceinline__ void contract_stress_form(const cx<T> (&v)[SM], const
   }
 }
 
+// contract_stress_form for v read from (and the result written back to)
+// LDS, io[b * stride]: the v_b are accumulated into s one at a time and the
+// outputs written once all of them are read, so only s, t, g and r stay in
+// registers.  No virtualevolv term (the GL steps).
+template <typename T, int S>
+__device__ __forceinline__ void contract_stress_form_lds(cx<T>* io, int stride, const ElasticConsts* __restrict__ ec,
+                                                         long pt) {
+  double f[3], g[6];
+  kvec(pt, ec, f);
+  if (!green6(f, ec, g)) {  // B(0) = 0
+#pragma unroll
+    for (int a = 0; a < S; a++) io[a * stride] = {(T)0, (T)0};
+    return;
+  }
+  cx<T> sv[6];
+#pragma unroll
+  for (int t = 0; t < 6; t++) sv[t] = {(T)0, (T)0};
+#pragma unroll
+  for (int b = 0; b < S; b++) {
+    const cx<T> vb = io[b * stride];
+    const T* sb = sig6_of<T>(ec, b);
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      sv[t].x += sb[t] * vb.x;
+      sv[t].y += sb[t] * vb.y;
+    }
+  }
+  const T f0 = (T)f[0], f1 = (T)f[1], f2 = (T)f[2];
+  cx<T> tv[3];
+  tv[0] = {sv[0].x * f0 + sv[1].x * f1 + sv[2].x * f2, sv[0].y * f0 + sv[1].y * f1 + sv[2].y * f2};
+  tv[1] = {sv[1].x * f0 + sv[3].x * f1 + sv[4].x * f2, sv[1].y * f0 + sv[3].y * f1 + sv[4].y * f2};
+  tv[2] = {sv[2].x * f0 + sv[4].x * f1 + sv[5].x * f2, sv[2].y * f0 + sv[4].y * f1 + sv[5].y * f2};
+  const T g0 = (T)g[0], g1 = (T)g[1], g2 = (T)g[2], g3 = (T)g[3], g4 = (T)g[4], g5 = (T)g[5];
+  cx<T> gv[3];
+  gv[0] = {g0 * tv[0].x + g1 * tv[1].x + g2 * tv[2].x, g0 * tv[0].y + g1 * tv[1].y + g2 * tv[2].y};
+  gv[1] = {g1 * tv[0].x + g3 * tv[1].x + g4 * tv[2].x, g1 * tv[0].y + g3 * tv[1].y + g4 * tv[2].y};
+  gv[2] = {g2 * tv[0].x + g4 * tv[1].x + g5 * tv[2].x, g2 * tv[0].y + g4 * tv[1].y + g5 * tv[2].y};
+  const T lam = (T)ec->ll, mu = (T)ec->mu, dg = (T)(2 * ec->mu + ec->mup);
+  const cx<T> y00 = {gv[0].x * f0, gv[0].y * f0}, y11 = {gv[1].x * f1, gv[1].y * f1},
+              y22 = {gv[2].x * f2, gv[2].y * f2};
+  const cx<T> tr = {lam * (y00.x + y11.x + y22.x), lam * (y00.y + y11.y + y22.y)};
+  cx<T> rv[6];
+  rv[0] = {sv[0].x - (tr.x + dg * y00.x), sv[0].y - (tr.y + dg * y00.y)};
+  rv[3] = {sv[3].x - (tr.x + dg * y11.x), sv[3].y - (tr.y + dg * y11.y)};
+  rv[5] = {sv[5].x - (tr.x + dg * y22.x), sv[5].y - (tr.y + dg * y22.y)};
+  rv[1] = {sv[1].x - mu * (gv[0].x * f1 + gv[1].x * f0), sv[1].y - mu * (gv[0].y * f1 + gv[1].y * f0)};
+  rv[2] = {sv[2].x - mu * (gv[0].x * f2 + gv[2].x * f0), sv[2].y - mu * (gv[0].y * f2 + gv[2].y * f0)};
+  rv[4] = {sv[4].x - mu * (gv[1].x * f2 + gv[2].x * f1), sv[4].y - mu * (gv[1].y * f2 + gv[2].y * f1)};
+#pragma unroll
+  for (int a = 0; a < S; a++) {
+    const T* ea = e6m_of<T>(ec, a);
+    cx<T> o = {(T)0, (T)0};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      o.x += ea[t] * rv[t].x;
+      o.y += ea[t] * rv[t].y;
+    }
+    io[a * stride] = o;
+  }
+}
+
 template <typename T, int SM, bool UNROLL_A, bool NTS = false>
 __device__ __forceinline__ void contract_point(const cx<T> (&v)[SM], const ElasticConsts* __restrict__ ec, long pt,
                                                int S, int quirk, bool stored, const T* __restrict__ Bsym, long P,
@@ -933,17 +994,19 @@ k_row_step_il(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
 }
 
 // Row kernel, interleaved layout, with S and the rows per block (R) fixed at
-// compile time, no frozen points and a uniform shear: one grid point per
-// thread, whose S master values are loaded before the inverse transform so
-// that they arrive while it runs, and whose tile elements are loaded (and
-// later stored) in one batch.  Per point and per pencil the same arithmetic
-// as k_row_step_il; each field's block sum adds the threads' values in
-// thread order.  grid = (row blocks, instances).
+// compile time, no frozen points and a uniform shear.  One TPP-lane group per
+// pencil (field f = p / R, row r = p % R), so each thread transforms one
+// pencil and updates the EPT = N / TPP points of it that it holds; its master
+// values are loaded before the inverse transform, so that they arrive while
+// it runs, and its share of the block's contiguous stretch of W is loaded
+// (and later stored) in one batch.  Per point and per pencil the same
+// arithmetic as k_row_step_il; a field's block sum adds its pencils' lanes in
+// a fixed butterfly.  grid = (row blocks, instances).
 template <typename T, int N, int S, int R> struct RowIlFixed {
-  static constexpr int TPP = make_plan(N).tpp, LD = pitch<N>(), NPEN = S * R;
-  static constexpr int NT = N * R;                  // one grid point per thread
-  static constexpr int PPI = NT / TPP;              // pencils transformed at once
-  static constexpr int G = (NPEN + PPI - 1) / PPI;  // pencils per TPP-thread group
+  static constexpr int TPP = make_plan(N).tpp, SH = 3, LD = padded_pitch<N, SH>(), NPEN = S * R;
+  static constexpr int NT = NPEN * TPP;             // one TPP-lane group per pencil
+  static constexpr int EPT = N / TPP;               // points per lane
+  static constexpr int EL = R * N * S / NT;         // stretch elements per thread (= EPT)
   static constexpr size_t LDS = (size_t)NPEN * LD * sizeof(cx<T>);
 };
 template <typename T, int N, int S, int R, typename M>
@@ -951,74 +1014,64 @@ __global__ void __launch_bounds__((RowIlFixed<T, N, S, R>::NT))
 k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict__ tw,
                double* __restrict__ partials, const int* __restrict__ step_ctr) {
   using K = RowIlFixed<T, N, S, R>;
-  constexpr int NT = K::NT, LD = K::LD, TPP = K::TPP, NW = NT / 64;
-  static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024, "tile shape");
+  constexpr int NT = K::NT, LD = K::LD, TPP = K::TPP, EPT = K::EPT, EL = K::EL, SH = K::SH;
+  static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024 && (R * N * S) % NT == 0, "tile shape");
+  static_assert(TPP <= 64 && 64 % TPP == 0, "a pencil's lanes lie in one wave");
   __shared__ cx<T> sm[K::NPEN * LD];
-  __shared__ double red[S][NW];
   const long P = a.P;
   const long nblk = P / ((long)N * R);
   const long blk = blockIdx.x;
   const int inst = blockIdx.y, t = threadIdx.x;
   const long pt0 = blk * R * N;
   cx<T>* wb = W + ((long)inst * P + pt0) * S;  // the block's R*N*S contiguous elements
-  // the thread's point: row r of the block, position i along N3
-  const int r = t / N, i = t % N;
-  double2 xv[S];
+  const int p = t / TPP, q = t % TPP, f = p / R, r = p % R;
+  const long xoff = ((long)inst * S + f) * P + pt0 + (long)r * N;  // this pencil's row of the master
+  double2 xv[EPT];
 #pragma unroll
-  for (int f = 0; f < S; f++) xv[f] = xi.load(((long)inst * S + f) * P + pt0 + t);
+  for (int j = 0; j < EPT; j++) xv[j] = xi.load(xoff + q + j * TPP);
   {
-    cx<T> buf[S];
+    cx<T> buf[EL];
 #pragma unroll
-    for (int k = 0; k < S; k++) buf[k] = wb[t + k * NT];
+    for (int k = 0; k < EL; k++) buf[k] = wb[t + k * NT];
 #pragma unroll
-    for (int k = 0; k < S; k++) {
-      const int e = t + k * NT, po = e / S, f = e % S;  // element e: point po, field f
-      sm[(f * R + po / N) * LD + po % N] = buf[k];     // pencil f * R + row
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, po = e / S, fe = e % S;  // element e: point po, field fe
+      sm[(fe * R + po / N) * LD + lds_pos<SH>(po % N)] = buf[k];
     }
   }
   __syncthreads();
-  const int p0 = t / TPP, q = t % TPP;
-  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)1, tw);
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, tw);
   const double inv_n = 1.0 / (double)P;
-  double acc[S];
+  const FieldCoef c = field_coef(a, inst * S + f, step_ctr);
+  double acc = 0.0;
 #pragma unroll
-  for (int f = 0; f < S; f++) {
-    const FieldCoef c = field_coef(a, inst * S + f, step_ctr);
-    cx<T>* cell = sm + (f * R + r) * LD + i;
+  for (int j = 0; j < EPT; j++) {
+    const int i = q + j * TPP;
+    cx<T>* cell = sm + p * LD + lds_pos<SH>(i);
     const cx<T> d = *cell;
-    double2 x = xv[f];
-    gl_point<false, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, nullptr, pt0 + t, inv_n);
-    xi.store(((long)inst * S + f) * P + pt0 + t, x);
+    double2 x = xv[j];
+    gl_point<false, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, nullptr, pt0 + (long)r * N + i, inv_n);
+    xi.store(xoff + i, x);
     *cell = {(T)x.x, (T)x.y};
-    acc[f] = x.x;  // (scaled by 1/P once per field)
+    acc += x.x;  // (scaled by 1/P once per field)
   }
   __syncthreads();
-  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)-1, tw);
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, tw);
   {
-    cx<T> buf[S];
+    cx<T> buf[EL];
 #pragma unroll
-    for (int k = 0; k < S; k++) {
-      const int e = t + k * NT, po = e / S, f = e % S;
-      buf[k] = sm[(f * R + po / N) * LD + po % N];
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, po = e / S, fe = e % S;
+      buf[k] = sm[(fe * R + po / N) * LD + lds_pos<SH>(po % N)];
     }
 #pragma unroll
-    for (int k = 0; k < S; k++) gstore<T>(wb + t + k * NT, buf[k]);
+    for (int k = 0; k < EL; k++) gstore<T>(wb + t + k * NT, buf[k]);
   }
-  // per-field block sums: each wave's lanes in a fixed butterfly, then the waves in order
+  // the field's R pencils are R * TPP consecutive lanes of one wave (TPP divides 64; R * TPP <= 64)
+  static_assert(R * TPP <= 64 && 64 % (R * TPP) == 0, "a field's lanes lie in one wave");
 #pragma unroll
-  for (int f = 0; f < S; f++) {
-    double v = acc[f];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((t & 63) == 0) red[f][t >> 6] = v;
-  }
-  __syncthreads();
-  if (t < S) {
-    double sum = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; w++) sum += red[t][w];
-    partials[((long)inst * S + t) * nblk + blk] = sum * inv_n;
-  }
+  for (int o = R * TPP / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((t % (R * TPP)) == 0) partials[((long)inst * S + f) * nblk + blk] = acc * inv_n;
 }
 
 template <typename T, int N> constexpr size_t row_il_lds_bytes(int S, int R, int nt) {
@@ -1065,66 +1118,65 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
   il_stage<T, EPT, false, false>(wb, sm, chunk, S, tile, goff, lidx);
 }
 
-// Column kernel, interleaved layout, with S, the tile (TC columns) and the
-// block (NT threads) fixed at compile time: a thread's loads of the tile are
-// all issued before its LDS stores, and its LDS reads before its stores to W,
-// so the whole tile is in flight at once; the k-points are contracted in the
-// stress-spectrum form; one k-point per thread.  Same arithmetic per k-point
-// and per pencil as k_col_contract_il.  grid = (column tiles, instances).
+// Column kernel, interleaved layout, with S and the tile (TC columns) fixed at
+// compile time, for the GL steps (no virtualevolv term): one TPP-lane group
+// per pencil (field a = p / TC, column c = p % TC); a thread's share of the
+// tile is loaded in one batch before its LDS stores, and stored in one batch
+// after its LDS reads; the k-points are contracted in the stress-spectrum
+// form.  Same arithmetic per k-point and per pencil as k_col_contract_il.
+// grid = (column tiles, instances).
 template <typename T, int N, int S, int TC> struct ColIlFixed {
-  static constexpr int TPP = make_plan(N).tpp, LD = pitch<N>(), NPEN = S * TC, ROW = S * TC;
-  static constexpr int NT = N * TC;                // one k-point per thread
-  static constexpr int EL = N * ROW / NT;          // tile elements per thread (= S)
-  static constexpr int PPI = NT / TPP;             // pencils transformed at once
-  static constexpr int G = (NPEN + PPI - 1) / PPI; // pencils per TPP-thread group
+  static constexpr int TPP = make_plan(N).tpp, SH = 3, LD = padded_pitch<N, SH>(), NPEN = S * TC, ROW = S * TC;
+  static constexpr int NT = NPEN * TPP;             // one TPP-lane group per pencil
+  static constexpr int EL = N * ROW / NT;           // tile elements per thread
   static constexpr size_t LDS = (size_t)NPEN * LD * sizeof(cx<T>);
 };
 template <typename T, int N, int S, int TC>
 __global__ void __launch_bounds__((ColIlFixed<T, N, S, TC>::NT))
-k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int quirk,
-               const cx<T>* __restrict__ tw) {
+k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, const cx<T>* __restrict__ tw) {
   using K = ColIlFixed<T, N, S, TC>;
-  constexpr int NT = K::NT, LD = K::LD, ROW = K::ROW, EL = K::EL, TPP = K::TPP;
-  static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024, "tile shape");
+  constexpr int NT = K::NT, LD = K::LD, ROW = K::ROW, EL = K::EL, TPP = K::TPP, SH = K::SH;
+  static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024 && (N * ROW) % NT == 0, "tile shape");
   __shared__ cx<T> sm[K::NPEN * LD];
   const long M = P / N;  // (i2, i3) columns
   const long m0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
   cx<T>* wb = W + ((long)blockIdx.y * P + m0) * S;  // row n of the tile at wb + n * M * S
   const long rs = M * S;
   const int t = threadIdx.x;
-  cx<T> buf[EL];
-#pragma unroll
-  for (int k = 0; k < EL; k++) {
-    const int e = t + k * NT, n = e / ROW, r = e % ROW;
-    buf[k] = wb[n * rs + r];
-  }
-#pragma unroll
-  for (int k = 0; k < EL; k++) {
-    const int e = t + k * NT, n = e / ROW, r = e % ROW;
-    sm[((r % S) * TC + r / S) * LD + n] = buf[k];  // pencil a * TC + c of element r = c * S + a
-  }
-  __syncthreads();
-  const int p0 = t / TPP, q = t % TPP;
-  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)-1, tw);
   {
-    const int c = t / N, n = t % N;  // consecutive threads read consecutive LDS words
-    cx<T> v[S];
+    cx<T> buf[EL];
 #pragma unroll
-    for (int b = 0; b < S; b++) v[b] = sm[(b * TC + c) * LD + n];
-    contract_point<T, S, false>(v, ec, (long)n * M + m0 + c, S, quirk, false, (const T*)nullptr, P,
-                                sm + c * LD + n, (long)TC * LD);
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, n = e / ROW, r = e % ROW;
+      buf[k] = wb[n * rs + r];
+    }
+#pragma unroll
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, n = e / ROW, r = e % ROW;
+      sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)] = buf[k];  // pencil a * TC + c of element r = c * S + a
+    }
   }
   __syncthreads();
-  fft_lds_multi<T, N, K::G, LD>(sm, p0, K::PPI, K::NPEN, q, (T)1, tw);
-#pragma unroll
-  for (int k = 0; k < EL; k++) {
-    const int e = t + k * NT, n = e / ROW, r = e % ROW;
-    buf[k] = sm[((r % S) * TC + r / S) * LD + n];
+  const int p = t / TPP, q = t % TPP;
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, tw);
+  for (int e = t; e < N * TC; e += NT) {
+    const int c = e / N, n = e % N;  // consecutive threads read consecutive LDS words
+    contract_stress_form_lds<T, S>(sm + c * LD + lds_pos<SH>(n), TC * LD, ec, (long)n * M + m0 + c);
   }
+  __syncthreads();
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, tw);
+  {
+    cx<T> buf[EL];
 #pragma unroll
-  for (int k = 0; k < EL; k++) {
-    const int e = t + k * NT, n = e / ROW, r = e % ROW;
-    gstore<T>(wb + n * rs + r, buf[k]);
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, n = e / ROW, r = e % ROW;
+      buf[k] = sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)];
+    }
+#pragma unroll
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, n = e / ROW, r = e % ROW;
+      gstore<T>(wb + n * rs + r, buf[k]);
+    }
   }
 }
 
