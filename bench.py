@@ -92,7 +92,9 @@ def measured_traffic(name):
     if not name or not os.path.exists(path):
         return None
     with open(path) as fh:
-        k = json.load(fh)["kernels"].get(name)
+        kernels = json.load(fh)["kernels"]
+    # rocprofv3 names carry the argument list after the template arguments
+    k = kernels.get(name) or next((v for key, v in kernels.items() if key.startswith(name + "(")), None)
     return None if k is None else int(round(k["traffic_bytes"]))
 
 
@@ -112,10 +114,16 @@ def step_alg_bytes(S, P, fp64, stored, lean=False):
 GRAPH_STEPS = 8
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def timed_steps(ctx, steps, warmup):
     """(ms for `steps` GL steps, host wall s, warm-up steps done): the warm-up
-    is at least GRAPH_STEPS steps, so the graph is captured before e0."""
-    wu = max(warmup, GRAPH_STEPS)
+    is at least 4 GRAPH_STEPS steps, so the graph is captured before e0 and
+    the clocks have settled (a 5-step warm-up left 20-step runs 5% slower
+    than long ones)."""
+    wu = max(warmup, 4 * GRAPH_STEPS)
     ctx.gl_steps(wu, record=False)
     ctx.sync()
     e0, e1 = hip.Event(), hip.Event()
@@ -225,6 +233,8 @@ def main():
     ap.add_argument("--bmode", choices=["recompute", "stored"], default="recompute")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-error-run", action="store_true",
+                    help="skip the 1000-step error comparison of the mixed modes (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the PFDD GL step is a single-GPU path here "
@@ -240,6 +250,7 @@ def main():
     ntri = S * (S + 1) // 2
 
     # ---- FP64 (value)
+    progress("FP64 timed steps")
     ctx = make_ctx(w, pa.FP64, bmode)
     ms64, wall64, wu = timed_steps(ctx, a.steps, a.warmup)
     stages = time_kernels(ctx)
@@ -284,6 +295,7 @@ def main():
     # ---- mixed precision: compensated FP32 master ("mixed"), plain FP32 master ("mixed_lean")
     modes = [] if a.no_mixed else [("mixed", pa.MIXED, False), ("mixed_lean", pa.MIXED_LEAN, True)]
     for key, prec, lean in modes:
+        progress(f"{key} timed steps")
         ctxm = make_ctx(w, prec, bmode)
         ms32, _, _ = timed_steps(ctxm, a.steps, a.warmup)
         stages32 = time_kernels(ctxm)
@@ -301,7 +313,9 @@ def main():
 
     # ---- errors of the mixed modes against FP64 over the workload's full 1000 GL steps (fresh
     # contexts, outside the timed region): xi, total slipped area, mean stress
-    if modes:
+    if modes and not a.no_error_run:
+        progress("error runs")
+
         def run_full(prec):
             c = make_ctx(w, prec, bmode)
             g, _ = c.gl_steps(w.steps)
@@ -325,6 +339,7 @@ def main():
         del xi64
 
     if not a.no_cpu_baseline:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)
 
