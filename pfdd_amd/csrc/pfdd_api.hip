@@ -87,7 +87,6 @@ struct pfdd_ctx {
   int il = 0;                // 1: W interleaved, [B][P][S] (fields of a point together); else [B][S][P]
   int il_fixed = 0;          // 1: the interleaved column kernel with compile-time S and tile (k_col_il_fixed)
   int il_fixed_row = 0;      // 1: the interleaved row kernel with compile-time S and rows (k_row_il_fixed)
-  int c3_grid = 0;           // blocks of the persistent k_col_il_fixed grid
   size_t c3_lds = 0;         // its dynamic LDS bytes
   int plan = 0;              // the applied PFDD_PLAN_* word
   int r4 = 0;                // bit ax: radix-4 sequence for the standalone passes along axis ax
@@ -472,39 +471,19 @@ int il_fixed_tile(int S, int n1, bool fp64) {
   if (S != 12 || n1 != 128) return 0;
   return fp64 ? 1 : 2;
 }
-// Blocks of the persistent k_col_il_fixed grid: as many as fit on the device
-// at once, a multiple of the 8 XCDs, no more than the tiles need (0 when the
-// kernel is not instantiated for this context).
-template <typename T, int N>
-int k3il_fixed_grid(pfdd_ctx* c, int& grid) {
-  constexpr int TC = sizeof(T) == 8 ? 1 : 2;
-  int dev = 0, ncu = 0, pc = 0;
-  HIPCK(hipGetDevice(&dev));
-  HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &pc, reinterpret_cast<const void*>(&k_col_il_fixed<T, N, 12, TC>), ColIlFixed<T, N, 12, TC>::NT, 0));
-  if (pc < 1) return PFDD_E_UNSUPPORTED;
-  const long ntiles = (long)c->B * (((long)c->p.n2 * c->p.n3) / TC);
-  long nb = std::min((long)pc * ncu, ((ntiles + NUM_XCD - 1) / NUM_XCD) * NUM_XCD);
-  grid = (int)std::max<long>(NUM_XCD, (nb / NUM_XCD) * NUM_XCD);
-  return PFDD_OK;
-}
 template <typename T, int N>
 void k3il_fixed(pfdd_ctx* c) {
   constexpr int TC = sizeof(T) == 8 ? 1 : 2;
-  const long ntiles = (long)c->B * (((long)c->p.n2 * c->p.n3) / TC);
-  hipLaunchKernelGGL((k_col_il_fixed<T, N, 12, TC>), dim3((unsigned)c->c3_grid), dim3(ColIlFixed<T, N, 12, TC>::NT),
-                     0, c->stream, (cx<T>*)c->W, c->ec, c->P, tw_of<T>(c->tw[0]), ntiles);
+  const long nb = ((long)c->p.n2 * c->p.n3) / TC;
+  hipLaunchKernelGGL((k_col_il_fixed<T, N, 12, TC>), dim3((unsigned)nb, (unsigned)c->B),
+                     dim3(ColIlFixed<T, N, 12, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P,
+                     tw_of<T>(c->tw[0]));
 }
 
 // Column kernel, interleaved layout (c->c3_tile columns of every field per block).
 template <typename T, int N>
 int k3il_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
   if constexpr (N == 128) {
-    if (attr_only && c->il_fixed) {
-      const int rc = k3il_fixed_grid<T, N>(c, c->c3_grid);
-      if (rc) return rc;
-    }
     if (!attr_only && c->il_fixed && !quirk) {  // (the prologue's virtualevolv term takes the generic kernel)
       k3il_fixed<T, N>(c);
       tap(c, PFDD_STAGE_CONTRACT);

@@ -1026,15 +1026,9 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
   cx<T>* wb = W + ((long)inst * P + pt0) * S;  // the block's R*N*S contiguous elements
   const int p = t / TPP, q = t % TPP, f = p / R, r = p % R;
   const long xoff = ((long)inst * S + f) * P + pt0 + (long)r * N;  // this pencil's row of the master
-  // FP64: the master values are loaded before the inverse transform, so they
-  // arrive while it runs; FP32: in the update loop (as in k_row_step, where
-  // the early loads measured slower in FP32)
-  constexpr bool EARLY = sizeof(T) == 8;
-  double2 xv[EARLY ? EPT : 1];
-  if constexpr (EARLY) {
+  double2 xv[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) xv[j] = xi.load(xoff + q + j * TPP);
-  }
+  for (int j = 0; j < EPT; j++) xv[j] = xi.load(xoff + q + j * TPP);
   {
     cx<T> buf[EL];
 #pragma unroll
@@ -1055,7 +1049,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
     const int i = q + j * TPP;
     cx<T>* cell = sm + p * LD + lds_pos<SH>(i);
     const cx<T> d = *cell;
-    double2 x = EARLY ? xv[EARLY ? j : 0] : xi.load(xoff + i);
+    double2 x = xv[j];
     gl_point<false, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, nullptr, pt0 + (long)r * N + i, inv_n);
     xi.store(xoff + i, x);
     *cell = {(T)x.x, (T)x.y};
@@ -1125,17 +1119,12 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
 }
 
 // Column kernel, interleaved layout, with S and the tile (TC columns) fixed at
-// compile time, for the GL steps (no virtualevolv term).  One TPP-lane group
-// per pencil (field a = p / TC, column c = p % TC); the k-points are
-// contracted in the stress-spectrum form straight from LDS.  The grid is
-// persistent and software-pipelined: each block runs a sequence of tiles, and
-// the loads of its next tile are issued before the current one is
-// transformed, so they are in flight during the FFT passes and the
-// contraction; a tile's stores drain while the next one computes.  Tiles go
-// to the XCDs in contiguous ranges, and neighbouring tiles of a range run at
-// the same time (their rows share cache lines).  Same arithmetic per k-point
-// and per pencil as k_col_contract_il.  ntiles = instances * (N2 N3 / TC);
-// gridDim.x is a multiple of 8.
+// compile time, for the GL steps (no virtualevolv term): one TPP-lane group
+// per pencil (field a = p / TC, column c = p % TC); a thread's share of the
+// tile is loaded in one batch before its LDS stores, and stored in one batch
+// after its LDS reads; the k-points are contracted in the stress-spectrum
+// form.  Same arithmetic per k-point and per pencil as k_col_contract_il.
+// grid = (column tiles, instances).
 template <typename T, int N, int S, int TC> struct ColIlFixed {
   static constexpr int TPP = make_plan(N).tpp, SH = 3, LD = padded_pitch<N, SH>(), NPEN = S * TC, ROW = S * TC;
   static constexpr int NT = NPEN * TPP;             // one TPP-lane group per pencil
@@ -1144,72 +1133,50 @@ template <typename T, int N, int S, int TC> struct ColIlFixed {
 };
 template <typename T, int N, int S, int TC>
 __global__ void __launch_bounds__((ColIlFixed<T, N, S, TC>::NT))
-k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, const cx<T>* __restrict__ tw,
-               long ntiles) {
+k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, const cx<T>* __restrict__ tw) {
   using K = ColIlFixed<T, N, S, TC>;
   constexpr int NT = K::NT, LD = K::LD, ROW = K::ROW, EL = K::EL, TPP = K::TPP, SH = K::SH;
   static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024 && (N * ROW) % NT == 0, "tile shape");
   __shared__ cx<T> sm[K::NPEN * LD];
-  const long M = P / N;       // (i2, i3) columns of an instance
-  const long tpi = M / TC;    // tiles per instance
-  const long rs = M * S;      // row stride of W
-  // this XCD's contiguous range of tiles, taken by its blocks in turn
-  const int x = blockIdx.x % NUM_XCD;
-  const long nslots = gridDim.x / NUM_XCD;
-  const long q0 = ntiles / NUM_XCD, r0 = ntiles % NUM_XCD;
-  const long begin = x * q0 + (x < r0 ? x : r0), count = q0 + (x < r0 ? 1 : 0);
-  long i = blockIdx.x / NUM_XCD;
-  if (i >= count) return;
-  const int t = threadIdx.x, p = t / TPP, q = t % TPP;
-  auto tile_base = [&](long tile, long& m0) {
-    const long inst = tile / tpi;
-    m0 = (tile - inst * tpi) * TC;
-    return W + (inst * P + m0) * S;  // row n of the tile at + n * rs
-  };
-  long m0 = 0;
-  cx<T>* wb = tile_base(begin + i, m0);
-  cx<T> buf[EL];
+  const long M = P / N;  // (i2, i3) columns
+  const long m0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
+  cx<T>* wb = W + ((long)blockIdx.y * P + m0) * S;  // row n of the tile at wb + n * M * S
+  const long rs = M * S;
+  const int t = threadIdx.x;
+  {
+    cx<T> buf[EL];
 #pragma unroll
-  for (int k = 0; k < EL; k++) {
-    const int e = t + k * NT, n = e / ROW, r = e % ROW;
-    buf[k] = wb[n * rs + r];
-  }
-  for (;;) {
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, n = e / ROW, r = e % ROW;
+      buf[k] = wb[n * rs + r];
+    }
 #pragma unroll
     for (int k = 0; k < EL; k++) {
       const int e = t + k * NT, n = e / ROW, r = e % ROW;
       sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)] = buf[k];  // pencil a * TC + c of element r = c * S + a
     }
-    __syncthreads();
-    // the next tile's loads, in flight while this one is transformed
-    const long inext = i + nslots;
-    long m0n = 0;
-    cx<T>* wn = nullptr;
-    if (inext < count) {
-      wn = tile_base(begin + inext, m0n);
-#pragma unroll
-      for (int k = 0; k < EL; k++) {
-        const int e = t + k * NT, n = e / ROW, r = e % ROW;
-        buf[k] = wn[n * rs + r];
-      }
-    }
-    fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, tw);
-    for (int e = t; e < N * TC; e += NT) {
-      const int c = e / N, n = e % N;  // consecutive threads read consecutive LDS words
-      contract_stress_form_lds<T, S>(sm + c * LD + lds_pos<SH>(n), TC * LD, ec, (long)n * M + m0 + c);
-    }
-    __syncthreads();
-    fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, tw);
+  }
+  __syncthreads();
+  const int p = t / TPP, q = t % TPP;
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, tw);
+  for (int e = t; e < N * TC; e += NT) {
+    const int c = e / N, n = e % N;  // consecutive threads read consecutive LDS words
+    contract_stress_form_lds<T, S>(sm + c * LD + lds_pos<SH>(n), TC * LD, ec, (long)n * M + m0 + c);
+  }
+  __syncthreads();
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, tw);
+  {
+    cx<T> buf[EL];
 #pragma unroll
     for (int k = 0; k < EL; k++) {
       const int e = t + k * NT, n = e / ROW, r = e % ROW;
-      gstore<T>(wb + n * rs + r, sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)]);
+      buf[k] = sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)];
     }
-    if (inext >= count) break;
-    __syncthreads();  // this tile's LDS reads are done before the next one is staged
-    i = inext;
-    wb = wn;
-    m0 = m0n;
+#pragma unroll
+    for (int k = 0; k < EL; k++) {
+      const int e = t + k * NT, n = e / ROW, r = e % ROW;
+      gstore<T>(wb + n * rs + r, buf[k]);
+    }
   }
 }
 
