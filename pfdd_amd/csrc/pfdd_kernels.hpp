@@ -281,11 +281,13 @@ __global__ void __launch_bounds__((tile_rows<T, N, RL>() * make_plan(N, RL).tpp)
 k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict__ tw) {
   constexpr int TPP = make_plan(N, RL).tpp, TILE = tile_rows<T, N, RL>(), LD = pitch<N>();
   __shared__ cx<T> sm[TILE * LD];
+  __shared__ cx<T> twl[N];
+  stage_twiddles<T, N>(twl, tw);
   const long row0 = (long)blockIdx.x * TILE;
   const int nr = (int)((nrows - row0) < TILE ? (nrows - row0) : TILE);
   const RowPencils<T> g{data + row0 * N, N};
   const LdsPencils<T> l{sm, LD};
-  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, tw);
+  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, twl);
 }
 
 // Transform along a strided axis of an [outer][N][inner] array; each block
@@ -298,13 +300,15 @@ k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* 
   constexpr int TILE = tile_cols<T, N, RL, WIDE>(), LD = pitch<N>();
   extern __shared__ unsigned char lds_raw[];  // cols_lds_bytes<T, N, RL, WIDE>()
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  __shared__ cx<T> twl[N];
+  stage_twiddles<T, N>(twl, tw);
   const long tiles_per_outer = inner / tile;
   const long bt = xcd_tile(blockIdx.x, gridDim.x);
   const long o = bt / tiles_per_outer;
   const long i0 = (bt % tiles_per_outer) * tile;
   const ColPencils<T> g{data + o * N * inner + i0, inner};
   const LdsPencils<T> l{sm, LD};
-  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, tw);
+  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, twl);
 }
 
 // ---------------------------------------------------------------- GL kernels
@@ -786,6 +790,8 @@ k_row_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
   static_assert(NT % 64 == 0, "block must be whole waves");
   __shared__ cx<T> sm[TILE * LD];
   __shared__ double red[NT / 64];
+  __shared__ cx<T> twl[N];
+  stage_twiddles<T, N>(twl, tw);
   const long P = a.P;
   const long rows_per_field = P / N;
   const long row0 = (long)blockIdx.x * tile;
@@ -810,7 +816,7 @@ k_row_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
       if (e < tile * N) xv[k] = xi.load(off + e);
     }
   }
-  fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, tw);
+  fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, twl);
   const double inv_n = 1.0 / (double)P;
   const FieldCoef c = field_coef(a, f, step_ctr);
   double acc = 0.0;
@@ -829,7 +835,7 @@ k_row_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
     acc += x.x;  // (the block sum is scaled by 1/P once)
   }
   __syncthreads();
-  fft_pencils<T, N, 1>(l, l, g, pen, 1 << 30, tile, q, (T)-1, tw);
+  fft_pencils<T, N, 1>(l, l, g, pen, 1 << 30, tile, q, (T)-1, twl);
   const double sum = block_sum(acc, red) * inv_n;
   if (threadIdx.x == 0) partials[(long)f * (rows_per_field / tile) + (pt0 / N) / tile] = sum;
 }
@@ -936,6 +942,8 @@ k_row_step_il(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   const int S = a.ns, npen = S * R, NT = blockDim.x;
   double* red = reinterpret_cast<double*>(sm + (long)npen * LD);
+  __shared__ cx<T> twl[N];
+  stage_twiddles<T, N>(twl, tw);
   const long P = a.P;
   const long nblk = P / ((long)N * R);
   const long blk = blockIdx.x;
@@ -951,7 +959,7 @@ k_row_step_il(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
   il_stage<T, EPT, true, BATCH>(wb, sm, chunk, S, N, goff, lidx);
   __syncthreads();
   const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
-  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, tw);
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, twl);
   const double inv_n = 1.0 / (double)P;
   double acc = 0.0;
   if (p < npen) {
@@ -989,7 +997,7 @@ k_row_step_il(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
     for (int k = 0; k < g; k++) s += red[threadIdx.x * g + k];
     partials[((long)inst * S + threadIdx.x) * nblk + blk] = s * inv_n;
   }
-  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)-1, tw);
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)-1, twl);
   il_stage<T, EPT, false, BATCH>(wb, sm, chunk, S, N, goff, lidx);
 }
 
@@ -1018,6 +1026,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
   static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024 && (R * N * S) % NT == 0, "tile shape");
   static_assert(TPP <= 64 && 64 % TPP == 0, "a pencil's lanes lie in one wave");
   __shared__ cx<T> sm[K::NPEN * LD];
+  __shared__ cx<T> twl[N];  // the twiddle table, read by every pass
   const long P = a.P;
   const long nblk = P / ((long)N * R);
   const long blk = blockIdx.x;
@@ -1029,6 +1038,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
   double2 xv[EPT];
 #pragma unroll
   for (int j = 0; j < EPT; j++) xv[j] = xi.load(xoff + q + j * TPP);
+  stage_twiddles<T, N>(twl, tw);
   {
     cx<T> buf[EL];
 #pragma unroll
@@ -1040,7 +1050,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
     }
   }
   __syncthreads();
-  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, tw);
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, twl);
   const double inv_n = 1.0 / (double)P;
   const FieldCoef c = field_coef(a, inst * S + f, step_ctr);
   double acc = 0.0;
@@ -1056,7 +1066,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
     acc += x.x;  // (scaled by 1/P once per field)
   }
   __syncthreads();
-  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, tw);
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, twl);
   {
     cx<T> buf[EL];
 #pragma unroll
@@ -1090,6 +1100,8 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
   constexpr int TPP = make_plan(N).tpp, LD = pitch<N>();
   extern __shared__ unsigned char lds_raw[];
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
+  __shared__ cx<T> twl[N];
+  stage_twiddles<T, N>(twl, tw);
   const int S = ec->ns, npen = S * tile, NT = blockDim.x;
   const long inner = P / N;
   const long m0 = xcd_tile(blockIdx.x, gridDim.x) * tile;
@@ -1103,7 +1115,7 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
   il_stage<T, EPT, true, false>(wb, sm, chunk, S, tile, goff, lidx);
   __syncthreads();
   const int p = threadIdx.x / TPP, q = threadIdx.x % TPP;
-  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)-1, tw);
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)-1, twl);
   for (int e = threadIdx.x; e < N * tile; e += NT) {
     const int c = e / N, n = e - c * N;  // consecutive threads read consecutive LDS words
     cx<T> v[SM];
@@ -1114,7 +1126,7 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
                                  sm + c * LD + n, (long)tile * LD);
   }
   __syncthreads();
-  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, tw);
+  fft_lds_multi<T, N, 1, LD>(sm, p, 1 << 30, npen, q, (T)1, twl);
   il_stage<T, EPT, false, false>(wb, sm, chunk, S, tile, goff, lidx);
 }
 
@@ -1138,11 +1150,13 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   constexpr int NT = K::NT, LD = K::LD, ROW = K::ROW, EL = K::EL, TPP = K::TPP, SH = K::SH;
   static_assert(NT % 64 == 0 && NT <= 1024 && K::LDS <= 64 * 1024 && (N * ROW) % NT == 0, "tile shape");
   __shared__ cx<T> sm[K::NPEN * LD];
+  __shared__ cx<T> twl[N];  // the twiddle table, read by every pass
   const long M = P / N;  // (i2, i3) columns
   const long m0 = xcd_tile(blockIdx.x, gridDim.x) * TC;
   cx<T>* wb = W + ((long)blockIdx.y * P + m0) * S;  // row n of the tile at wb + n * M * S
   const long rs = M * S;
   const int t = threadIdx.x;
+  stage_twiddles<T, N>(twl, tw);
   {
     cx<T> buf[EL];
 #pragma unroll
@@ -1158,13 +1172,13 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   }
   __syncthreads();
   const int p = t / TPP, q = t % TPP;
-  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, tw);
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, twl);
   for (int e = t; e < N * TC; e += NT) {
     const int c = e / N, n = e % N;  // consecutive threads read consecutive LDS words
     contract_stress_form_lds<T, S>(sm + c * LD + lds_pos<SH>(n), TC * LD, ec, (long)n * M + m0 + c);
   }
   __syncthreads();
-  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, tw);
+  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, twl);
   {
     cx<T> buf[EL];
 #pragma unroll
