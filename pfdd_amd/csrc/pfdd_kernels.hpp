@@ -1135,7 +1135,10 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
 // per pencil (field a = p / TC, column c = p % TC); a thread's share of the
 // tile is loaded in one batch before its LDS stores, and stored in one batch
 // after its LDS reads; the k-points are contracted in the stress-spectrum
-// form.  Same arithmetic per k-point and per pencil as k_col_contract_il.
+// form.  The stores are ordinary (not streaming): a tile's row segments
+// (S * TC complex) share cache lines with the neighbouring tiles, and
+// non-temporal stores of them wrote 12% more bytes than the data.  Same
+// arithmetic per k-point and per pencil as k_col_contract_il.
 // grid = (column tiles, instances).
 template <typename T, int N, int S, int TC> struct ColIlFixed {
   static constexpr int TPP = make_plan(N).tpp, SH = 3, LD = padded_pitch<N, SH>(), NPEN = S * TC, ROW = S * TC;
@@ -1189,7 +1192,7 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
 #pragma unroll
     for (int k = 0; k < EL; k++) {
       const int e = t + k * NT, n = e / ROW, r = e % ROW;
-      gstore<T>(wb + n * rs + r, buf[k]);
+      wb[n * rs + r] = buf[k];
     }
   }
 }
