@@ -312,6 +312,31 @@ __device__ __forceinline__ void stage_twiddles(cx<T>* twl, const cx<T>* __restri
   for (int k = threadIdx.x; k < N; k += blockDim.x) twl[k] = tw[k];
 }
 
+// The same copy split in two, for kernels that load a tile into LDS at the
+// start: load() issues this thread's global loads of the table, store()
+// writes them to LDS.  Issued after the tile's loads and stored with them,
+// the table's latency overlaps the tile's instead of preceding it (the
+// store of stage_twiddles waits for its load, and the in-order load counter
+// makes that wait cover every load issued before it).
+template <typename T, int N, int NT> struct TwStage {
+  static constexpr int K = (N + NT - 1) / NT;
+  cx<T> v[K];
+  __device__ __forceinline__ void load(const cx<T>* __restrict__ tw) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (N % NT == 0 || i < N) v[k] = tw[i];
+    }
+  }
+  __device__ __forceinline__ void store(cx<T>* twl) const {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (N % NT == 0 || i < N) twl[i] = v[k];
+    }
+  }
+};
+
 // LDS row pitch (in complex elements) for a pencil of N points.
 template <int N> constexpr int pitch() { return N + 1; }
 

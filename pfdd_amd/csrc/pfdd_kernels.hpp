@@ -35,12 +35,18 @@ template <typename T> struct Master;
 // step keep more of L2.  Non-temporal stores with ordinary loads: 0.978 ms).
 template <typename V> __device__ __forceinline__ V xld(const V* p) { return __builtin_nontemporal_load(p); }
 template <typename V> __device__ __forceinline__ void xst(V* p, V v) { __builtin_nontemporal_store(v, p); }
+// raw(i) loads a point in its stored form and widen() turns that into the
+// FP64 (Re, Im) the update works on: a kernel that loads its points long
+// before the update holds the narrower raw form in registers meanwhile.
 template <> struct Master<double> {
   double2* p;
-  __device__ __forceinline__ double2 load(long i) const {
+  using Raw = double2;
+  __device__ __forceinline__ Raw raw(long i) const {
     const VecOf<double>::t v = xld(reinterpret_cast<const VecOf<double>::t*>(p + i));
     return make_double2(v.x, v.y);
   }
+  static __device__ __forceinline__ double2 widen(Raw v) { return v; }
+  __device__ __forceinline__ double2 load(long i) const { return raw(i); }
   __device__ __forceinline__ void store(long i, double2 v) const {
     VecOf<double>::t w;
     w.x = v.x;
@@ -55,10 +61,17 @@ template <> struct Master<double> {
 struct MasterF32C {  // compensated: (hi, lo) of Re xi, and Im xi
   float2* r;
   float* im;
-  __device__ __forceinline__ double2 load(long i) const {
-    const VecOf<float>::t h = xld(reinterpret_cast<const VecOf<float>::t*>(r + i));
-    return make_double2((double)h.x + (double)h.y, (double)xld(im + i));
+  struct Raw {
+    VecOf<float>::t h;
+    float im;
+  };
+  __device__ __forceinline__ Raw raw(long i) const {
+    return Raw{xld(reinterpret_cast<const VecOf<float>::t*>(r + i)), xld(im + i)};
   }
+  static __device__ __forceinline__ double2 widen(Raw v) {
+    return make_double2((double)v.h.x + (double)v.h.y, (double)v.im);
+  }
+  __device__ __forceinline__ double2 load(long i) const { return widen(raw(i)); }
   __device__ __forceinline__ void store(long i, double2 v) const {
     const float hi = (float)v.x;
     VecOf<float>::t w;
@@ -74,10 +87,10 @@ struct MasterF32C {  // compensated: (hi, lo) of Re xi, and Im xi
 };
 struct MasterF32L {  // lean: (Re xi, Im xi) as a plain FP32 complex
   float2* r;
-  __device__ __forceinline__ double2 load(long i) const {
-    const VecOf<float>::t h = xld(reinterpret_cast<const VecOf<float>::t*>(r + i));
-    return make_double2((double)h.x, (double)h.y);
-  }
+  using Raw = VecOf<float>::t;
+  __device__ __forceinline__ Raw raw(long i) const { return xld(reinterpret_cast<const VecOf<float>::t*>(r + i)); }
+  static __device__ __forceinline__ double2 widen(Raw h) { return make_double2((double)h.x, (double)h.y); }
+  __device__ __forceinline__ double2 load(long i) const { return widen(raw(i)); }
   __device__ __forceinline__ void store(long i, double2 v) const {
     VecOf<float>::t w;
     w.x = (float)v.x;
@@ -91,9 +104,12 @@ struct MasterF32L {  // lean: (Re xi, Im xi) as a plain FP32 complex
 template <> struct Master<float> {
   float2* r;  // (hi, lo) of Re xi; (Re xi, Im xi) in the lean form
   float* im;  // Im xi; null in the lean form
+  using Raw = double2;
   __device__ __forceinline__ double2 load(long i) const {
     return im ? MasterF32C{r, im}.load(i) : MasterF32L{r}.load(i);
   }
+  __device__ __forceinline__ Raw raw(long i) const { return load(i); }
+  static __device__ __forceinline__ double2 widen(Raw v) { return v; }
   __device__ __forceinline__ void store(long i, double2 v) const {
     if (im) MasterF32C{r, im}.store(i, v);
     else MasterF32L{r}.store(i, v);
@@ -1035,19 +1051,29 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
   cx<T>* wb = W + ((long)inst * P + pt0) * S;  // the block's R*N*S contiguous elements
   const int p = t / TPP, q = t % TPP, f = p / R, r = p % R;
   const long xoff = ((long)inst * S + f) * P + pt0 + (long)r * N;  // this pencil's row of the master
-  double2 xv[EPT];
-#pragma unroll
-  for (int j = 0; j < EPT; j++) xv[j] = xi.load(xoff + q + j * TPP);
-  stage_twiddles<T, N>(twl, tw);
+  // Loads in flight together: the stretch of W, the twiddle table, and the
+  // thread's master points, which the update uses after the inverse
+  // transform (the LDS stores below wait only for the first two).
+  TwStage<T, N, NT> ts;
+  typename M::Raw xv[EPT];
+  // Element t + k*NT of the stretch is point po0 + k*KP of field fe: with
+  // NT a multiple of S, the thread's field is the same for every k, and
+  // with KP a multiple of 2^SH the padded LDS slot moves by a constant.
+  constexpr int KP = NT / S;
+  static_assert(NT % S == 0 && KP % (1 << SH) == 0 && N % KP == 0, "stretch walk");
+  const int po0 = t / S, fe = t % S;
+  cx<T>* lrow = sm + (fe * R + po0 / N) * LD + lds_pos<SH>(po0 % N);
+  auto lslot = [](int k) { return (k * KP) / N * LD + (k * KP) % N + ((k * KP) % N >> SH); };
   {
     cx<T> buf[EL];
 #pragma unroll
     for (int k = 0; k < EL; k++) buf[k] = wb[t + k * NT];
+    ts.load(tw);
 #pragma unroll
-    for (int k = 0; k < EL; k++) {
-      const int e = t + k * NT, po = e / S, fe = e % S;  // element e: point po, field fe
-      sm[(fe * R + po / N) * LD + lds_pos<SH>(po % N)] = buf[k];
-    }
+    for (int j = 0; j < EPT; j++) xv[j] = xi.raw(xoff + q + j * TPP);
+#pragma unroll
+    for (int k = 0; k < EL; k++) lrow[lslot(k)] = buf[k];
+    ts.store(twl);
   }
   __syncthreads();
   fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, twl);
@@ -1059,7 +1085,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
     const int i = q + j * TPP;
     cx<T>* cell = sm + p * LD + lds_pos<SH>(i);
     const cx<T> d = *cell;
-    double2 x = xv[j];
+    double2 x = M::widen(xv[j]);
     gl_point<false, lean_master<M>()>(x, (double)d.x, (double)d.y, c, a, nullptr, pt0 + (long)r * N + i, inv_n);
     xi.store(xoff + i, x);
     *cell = {(T)x.x, (T)x.y};
@@ -1070,10 +1096,7 @@ k_row_il_fixed(cx<T>* __restrict__ W, M xi, StepArgs a, const cx<T>* __restrict_
   {
     cx<T> buf[EL];
 #pragma unroll
-    for (int k = 0; k < EL; k++) {
-      const int e = t + k * NT, po = e / S, fe = e % S;
-      buf[k] = sm[(fe * R + po / N) * LD + lds_pos<SH>(po % N)];
-    }
+    for (int k = 0; k < EL; k++) buf[k] = lrow[lslot(k)];
 #pragma unroll
     for (int k = 0; k < EL; k++) gstore<T>(wb + t + k * NT, buf[k]);
   }
@@ -1142,12 +1165,17 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
 // grid = (column tiles, instances).
 template <typename T, int N, int S, int TC> struct ColIlFixed {
   static constexpr int TPP = make_plan(N).tpp, SH = 3, LD = padded_pitch<N, SH>(), NPEN = S * TC, ROW = S * TC;
+  // Minimum waves per SIMD asked of the register allocator: the FP32 form
+  // fits in the 72 registers that seven waves allow without spilling (at
+  // N = 128 four blocks per CU instead of three: 0.124 -> 0.102 ms at
+  // 128^3 x 12); the FP64 form is bound by LDS.
+  static constexpr int MINW = sizeof(T) == 4 ? 6 : 1;
   static constexpr int NT = NPEN * TPP;             // one TPP-lane group per pencil
   static constexpr int EL = N * ROW / NT;           // tile elements per thread
   static constexpr size_t LDS = (size_t)NPEN * LD * sizeof(cx<T>);
 };
 template <typename T, int N, int S, int TC>
-__global__ void __launch_bounds__((ColIlFixed<T, N, S, TC>::NT))
+__global__ void __launch_bounds__((ColIlFixed<T, N, S, TC>::NT), (ColIlFixed<T, N, S, TC>::MINW))
 k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, const cx<T>* __restrict__ tw) {
   using K = ColIlFixed<T, N, S, TC>;
   constexpr int NT = K::NT, LD = K::LD, ROW = K::ROW, EL = K::EL, TPP = K::TPP, SH = K::SH;
@@ -1159,19 +1187,23 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   cx<T>* wb = W + ((long)blockIdx.y * P + m0) * S;  // row n of the tile at wb + n * M * S
   const long rs = M * S;
   const int t = threadIdx.x;
-  stage_twiddles<T, N>(twl, tw);
+  // Tile element t + k*NT is row n0 + k*KN, offset r in the row: with NT a
+  // multiple of ROW the thread's offset r (column c = r / S, field a = r % S)
+  // is the same for every k.  Row offsets stay below N * rs = P * S < 2^31.
+  constexpr int KN = NT / ROW;
+  static_assert(NT % ROW == 0 && KN % (1 << SH) == 0, "tile walk");
+  const int n0 = t / ROW, r = t % ROW;
+  const cx<T>* gsrc = wb + (unsigned)(n0 * (int)rs + r);
+  cx<T>* lcol = sm + ((r % S) * TC + r / S) * LD + lds_pos<SH>(n0);  // pencil a * TC + c
   {
     cx<T> buf[EL];
 #pragma unroll
-    for (int k = 0; k < EL; k++) {
-      const int e = t + k * NT, n = e / ROW, r = e % ROW;
-      buf[k] = wb[n * rs + r];
-    }
+    for (int k = 0; k < EL; k++) buf[k] = gsrc[(unsigned)(k * KN * (int)rs)];
+    TwStage<T, N, NT> ts;  // (the twiddle loads after the tile's, stored with it)
+    ts.load(tw);
 #pragma unroll
-    for (int k = 0; k < EL; k++) {
-      const int e = t + k * NT, n = e / ROW, r = e % ROW;
-      sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)] = buf[k];  // pencil a * TC + c of element r = c * S + a
-    }
+    for (int k = 0; k < EL; k++) lcol[k * (KN + (KN >> SH))] = buf[k];
+    ts.store(twl);
   }
   __syncthreads();
   const int p = t / TPP, q = t % TPP;
@@ -1185,15 +1217,10 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   {
     cx<T> buf[EL];
 #pragma unroll
-    for (int k = 0; k < EL; k++) {
-      const int e = t + k * NT, n = e / ROW, r = e % ROW;
-      buf[k] = sm[((r % S) * TC + r / S) * LD + lds_pos<SH>(n)];
-    }
+    for (int k = 0; k < EL; k++) buf[k] = lcol[k * (KN + (KN >> SH))];
+    cx<T>* gdst = wb + (unsigned)(n0 * (int)rs + r);
 #pragma unroll
-    for (int k = 0; k < EL; k++) {
-      const int e = t + k * NT, n = e / ROW, r = e % ROW;
-      wb[n * rs + r] = buf[k];
-    }
+    for (int k = 0; k < EL; k++) gdst[(unsigned)(k * KN * (int)rs)] = buf[k];
   }
 }
 
