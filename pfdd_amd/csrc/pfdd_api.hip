@@ -370,12 +370,14 @@ int k1il_launch_n(pfdd_ctx* c, bool attr_only) {
   return rc;
 }
 
+// Lengths with compile-time interleaved row and column kernels (S = 12).
+constexpr bool il_fixed_len(int n) { return n == 128 || n == 256; }
 // Rows per block of the compile-time interleaved row kernel (k_row_il_fixed)
 // for ns fields along n3, or 0 where it is not instantiated.
-int il_fixed_rows(int S, int n3) { return S == 12 && n3 == 128 ? 1 : 0; }
+int il_fixed_rows(int S, int n3) { return S == 12 && il_fixed_len(n3) ? 1 : 0; }
 template <typename T, int N>
 bool k1il_fixed(pfdd_ctx* c) {
-  if constexpr (N == 128) {
+  if constexpr (il_fixed_len(N)) {
     if (!c->il_fixed_row || c->mask || c->taupp) return false;
     with_master<T>(c, [&](auto xi) {
       using M = decltype(xi);
@@ -468,7 +470,7 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
 // Tile of the compile-time interleaved column kernel (k_col_il_fixed) for
 // ns fields along n1, or 0 where it is not instantiated.
 int il_fixed_tile(int S, int n1, bool fp64) {
-  if (S != 12 || n1 != 128) return 0;
+  if (S != 12 || !il_fixed_len(n1)) return 0;
   return fp64 ? 1 : 2;
 }
 template <typename T, int N>
@@ -483,7 +485,7 @@ void k3il_fixed(pfdd_ctx* c) {
 // Column kernel, interleaved layout (c->c3_tile columns of every field per block).
 template <typename T, int N>
 int k3il_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
-  if constexpr (N == 128) {
+  if constexpr (il_fixed_len(N)) {
     if (!attr_only && c->il_fixed && !quirk) {  // (the prologue's virtualevolv term takes the generic kernel)
       k3il_fixed<T, N>(c);
       tap(c, PFDD_STAGE_CONTRACT);
@@ -922,12 +924,17 @@ int apply_plan(pfdd_ctx* c, int plan) {
   size_t c3_lds = 0;
   if ((plan & PFDD_PLAN_INTERLEAVED) && (plan & PFDD_PLAN_FIELD_MAJOR)) return PFDD_E_SHAPE;
   // AUTO takes the interleaved layout where its compile-time row and column
-  // kernels exist (S = 12 at N1 = N3 = 128, recomputed B): 0.749 against
-  // 0.918 ms per FP64 step and 0.461 against 0.490 ms per mixed step at
-  // 128^3 x 12.  Without them the field-major plans measured faster.
+  // kernels exist (S = 12 at N1 = N3 = 128 or 256, recomputed B) and
+  // measured faster than the field-major plan: at 128^3 x 12 0.686 against
+  // 0.909 ms per FP64 step and 0.400 against 0.466 ms per mixed step; at
+  // 256^3 x 12 3.82 against 4.52 ms (mixed) and 3.43 against 3.80 ms (lean
+  // mixed), but 8.60 against 7.54 ms in FP64, where the row kernel's 60 KB
+  // of LDS leave two blocks per CU, so FP64 at 256 stays field-major.
+  // Without the compile-time kernels the field-major plans measured faster.
   const bool auto_il = !(plan & (PFDD_PLAN_FIELD_MAJOR | PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS |
                                  PFDD_PLAN_FUSED_COLS | PFDD_PLAN_PLANE | PFDD_PLAN_NO_PLANE)) &&
-                       !c->stored && il_fixed_tile(S, p.n1, c->fp64 != 0) > 0 && il_fixed_rows(S, p.n3) > 0;
+                       !c->stored && il_fixed_tile(S, p.n1, c->fp64 != 0) > 0 && il_fixed_rows(S, p.n3) > 0 &&
+                       (!c->fp64 || (p.n1 == 128 && p.n3 == 128));
   if ((plan & PFDD_PLAN_INTERLEAVED) || auto_il) {
     // interleaved layout: the interleaved row kernel, no plane kernel
     if (!fused_len(p.n3) || (plan & (PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_PLANE))) return PFDD_E_SHAPE;

@@ -208,9 +208,14 @@ template <typename T> struct LdsColPencils {  // columns of a row-major LDS plan
 // pencil.  tw[m] = exp(+2 pi i m / N); the sign s flips its imaginary part.
 // Every thread of the block calls it: there is a barrier between the read
 // and the write phase, and one after the write phase when out is in LDS.
-template <typename T, int N, int R, int NSB, int TPP, int G, typename In, typename Out>
+// `hook` runs after the read phase, before the barrier (the kernels that
+// read the first pass from HBM store their twiddle table there; see TwStage).
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+template <typename T, int N, int R, int NSB, int TPP, int G, typename In, typename Out, typename Hook = NoHook>
 __device__ __forceinline__ void stockham_pass(const In& in, const Out& out, int p0, int pstride, int npen, int q,
-                                              T s, const cx<T>* __restrict__ tw) {
+                                              T s, const cx<T>* __restrict__ tw, const Hook& hook = Hook{}) {
   constexpr int M = N / R;
   constexpr int J = (M + TPP - 1) / TPP;
   constexpr int STEP = N / (NSB * R);
@@ -238,6 +243,7 @@ __device__ __forceinline__ void stockham_pass(const In& in, const Out& out, int 
       }
     }
   }
+  hook();
   if (In::in_lds || Out::in_lds) __syncthreads();
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -256,16 +262,17 @@ __device__ __forceinline__ void stockham_pass(const In& in, const Out& out, int 
   if (Out::in_lds) __syncthreads();
 }
 
-template <typename T, int N, int G, int K, int RL, typename In, typename Mid, typename Out>
+template <typename T, int N, int G, int K, int RL, typename In, typename Mid, typename Out, typename Hook = NoHook>
 __device__ __forceinline__ void fft_from(const In& in, const Mid& lds, const Out& out, int p0, int pstride,
-                                         int npen, int q, T s, const cx<T>* __restrict__ tw) {
+                                         int npen, int q, T s, const cx<T>* __restrict__ tw,
+                                         const Hook& hook = Hook{}) {
   constexpr Plan pl = make_plan(N, RL);
   if constexpr (K < pl.np) {
     constexpr int R = pl.r[K], NSB = ns_before(N, K, RL), TPP = pl.tpp;
     if constexpr (pl.np == 1) {
-      stockham_pass<T, N, R, NSB, TPP, G>(in, out, p0, pstride, npen, q, s, tw);
+      stockham_pass<T, N, R, NSB, TPP, G>(in, out, p0, pstride, npen, q, s, tw, hook);
     } else if constexpr (K == 0) {
-      stockham_pass<T, N, R, NSB, TPP, G>(in, lds, p0, pstride, npen, q, s, tw);
+      stockham_pass<T, N, R, NSB, TPP, G>(in, lds, p0, pstride, npen, q, s, tw, hook);
     } else if constexpr (K == pl.np - 1) {
       stockham_pass<T, N, R, NSB, TPP, G>(lds, out, p0, pstride, npen, q, s, tw);
     } else {
@@ -280,10 +287,11 @@ __device__ __forceinline__ void fft_from(const In& in, const Mid& lds, const Out
 // `lds` (which needs npen pencils of pitch >= N).  When the input is already
 // in LDS the caller synchronises first; the function ends synchronised when
 // the output is in LDS.
-template <typename T, int N, int G, int RL = 8, typename In, typename Mid, typename Out>
+template <typename T, int N, int G, int RL = 8, typename In, typename Mid, typename Out, typename Hook = NoHook>
 __device__ __forceinline__ void fft_pencils(const In& in, const Mid& lds, const Out& out, int p0,
-                                            int pstride, int npen, int q, T s, const cx<T>* __restrict__ tw) {
-  fft_from<T, N, G, 0, RL>(in, lds, out, p0, pstride, npen, q, s, tw);
+                                            int pstride, int npen, int q, T s, const cx<T>* __restrict__ tw,
+                                            const Hook& hook = Hook{}) {
+  fft_from<T, N, G, 0, RL>(in, lds, out, p0, pstride, npen, q, s, tw, hook);
 }
 
 // All-LDS transforms of up to G pencils per thread (pencil p at sm + p*LD,

@@ -298,12 +298,15 @@ k_fft_rows(cx<T>* __restrict__ data, long nrows, T sign, const cx<T>* __restrict
   constexpr int TPP = make_plan(N, RL).tpp, TILE = tile_rows<T, N, RL>(), LD = pitch<N>();
   __shared__ cx<T> sm[TILE * LD];
   __shared__ cx<T> twl[N];
-  stage_twiddles<T, N>(twl, tw);
+  // the table is stored to LDS after the first pass's HBM reads (which use no twiddles) are issued
+  TwStage<T, N, TILE * TPP> ts;
+  ts.load(tw);
   const long row0 = (long)blockIdx.x * TILE;
   const int nr = (int)((nrows - row0) < TILE ? (nrows - row0) : TILE);
   const RowPencils<T> g{data + row0 * N, N};
   const LdsPencils<T> l{sm, LD};
-  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, twl);
+  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x / TPP, 1 << 30, nr, (int)threadIdx.x % TPP, sign, twl,
+                           [&] { ts.store(twl); });
 }
 
 // Transform along a strided axis of an [outer][N][inner] array; each block
@@ -317,14 +320,17 @@ k_fft_cols(cx<T>* __restrict__ data, long inner, int tile, T sign, const cx<T>* 
   extern __shared__ unsigned char lds_raw[];  // cols_lds_bytes<T, N, RL, WIDE>()
   cx<T>* sm = reinterpret_cast<cx<T>*>(lds_raw);
   __shared__ cx<T> twl[N];
-  stage_twiddles<T, N>(twl, tw);
+  // the table is stored to LDS after the first pass's HBM reads (which use no twiddles) are issued
+  TwStage<T, N, cols_threads<T, N, RL, WIDE>()> ts;
+  ts.load(tw);
   const long tiles_per_outer = inner / tile;
   const long bt = xcd_tile(blockIdx.x, gridDim.x);
   const long o = bt / tiles_per_outer;
   const long i0 = (bt % tiles_per_outer) * tile;
   const ColPencils<T> g{data + o * N * inner + i0, inner};
   const LdsPencils<T> l{sm, LD};
-  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, twl);
+  fft_pencils<T, N, 1, RL>(g, l, g, (int)threadIdx.x % TILE, 1 << 30, tile, (int)threadIdx.x / TILE, sign, twl,
+                           [&] { ts.store(twl); });
 }
 
 // ---------------------------------------------------------------- GL kernels
@@ -807,7 +813,8 @@ k_row_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
   __shared__ cx<T> sm[TILE * LD];
   __shared__ double red[NT / 64];
   __shared__ cx<T> twl[N];
-  stage_twiddles<T, N>(twl, tw);
+  TwStage<T, N, NT> ts;  // stored to LDS after the first pass's HBM reads are issued
+  ts.load(tw);
   const long P = a.P;
   const long rows_per_field = P / N;
   const long row0 = (long)blockIdx.x * tile;
@@ -832,7 +839,7 @@ k_row_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
       if (e < tile * N) xv[k] = xi.load(off + e);
     }
   }
-  fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, twl);
+  fft_pencils<T, N, 1>(g, l, l, pen, 1 << 30, tile, q, (T)1, twl, [&] { ts.store(twl); });
   const double inv_n = 1.0 / (double)P;
   const FieldCoef c = field_coef(a, f, step_ctr);
   double acc = 0.0;
