@@ -44,8 +44,18 @@ enum {
  * FP64 update on a compensated FP32 master xi (12 B a point, Re xi to ~48
  * bits).  PFDD_MIXED_LEAN: the same with xi kept as a plain FP32 complex
  * (8 B a point), rounded to FP32 after every update, and the update's sine
- * taken in FP32 after an FP64 reduction of its argument mod 2 pi. */
-enum { PFDD_FP64 = 0, PFDD_MIXED = 1, PFDD_MIXED_LEAN = 2 };
+ * taken in FP32 after an FP64 reduction of its argument mod 2 pi.
+ * PFDD_REF32: the reference's own float arithmetic, for checks against the
+ * compiled reference: float data/data2 arrays (G.c:61-63), the FOURN
+ * transform bit for bit (G.c:2756-2827; the axes N3, N2, N1 in both
+ * directions), data2 accumulated as float sums of double products
+ * (G.c:4609-4629, virtualevolv's terms G.c:1426-1447), the FP64 master xi
+ * (G.c:4146) and the float uq2*data2/nsize of plasticevolv (G.c:3486-3489).
+ * Power-of-two edges only (PFDD_E_RADIX otherwise), the unfused
+ * field-major kernels only (PFDD_E_SHAPE for other plan flags), B(k) formed
+ * in FP64 at each k-point in either bmode; no ensembles and no autotuner
+ * (PFDD_E_UNSUPPORTED). */
+enum { PFDD_FP64 = 0, PFDD_MIXED = 1, PFDD_MIXED_LEAN = 2, PFDD_REF32 = 3 };
 enum { PFDD_B_STORED = 0, PFDD_B_RECOMPUTE = 1 }; /* interaction matrix B(k) */
 enum { PFDD_FORWARD = -1, PFDD_BACKWARD = 1 };    /* FFT sign (G.c:106, FFTW_FORWARD) */
 
@@ -54,7 +64,7 @@ enum { PFDD_FORWARD = -1, PFDD_BACKWARD = 1 };    /* FFT sign (G.c:106, FFTW_FOR
 typedef struct {
   int n1, n2, n3;              /* grid; each edge 2^a 3^b, 1 < n <= 1024 */
   int ns;                      /* active slip systems, 1..PFDD_MAX_NS */
-  int precision;               /* PFDD_FP64 | PFDD_MIXED | PFDD_MIXED_LEAN */
+  int precision;               /* PFDD_FP64 | PFDD_MIXED | PFDD_MIXED_LEAN | PFDD_REF32 */
   int bmode;                   /* PFDD_B_STORED | PFDD_B_RECOMPUTE */
   double C11, C12, C44;        /* elastic constants; mu = C44 (G.c:337) */
   double d1, d2, d3;           /* grid spacing of frec (G.c:4266-4268) */
@@ -174,7 +184,9 @@ int pfdd_strain_stress(pfdd_ctx *c, const double *avepst_uniform, double *aveps,
 /* Raw FFT seam (fft_forward / fft_backward, G.c:209-273): batched, in place,
  * unnormalised 3D C2C on a device buffer of batch contiguous N1*N2*N3 complex
  * blocks.  dev_buf is double2 when prec == PFDD_FP64, float2 when
- * prec == PFDD_MIXED (PFDD_E_SHAPE for other values).
+ * prec == PFDD_MIXED (PFDD_E_SHAPE for other values), or float2 transformed
+ * as fourn does it, bit for bit, when prec == PFDD_REF32 on a PFDD_REF32
+ * context (PFDD_E_STATE on others).
  * dir = PFDD_FORWARD (-1) or PFDD_BACKWARD (+1).  Asynchronous. */
 int pfdd_fft(pfdd_ctx *c, void *dev_buf, int batch, int dir, int prec);
 
@@ -345,7 +357,10 @@ int pfdd_plastic_converge(double gamma, double gammalast, int it_plastic, int *t
  * fft_forward / fft_backward have the exact signature and semantics of
  * G.c:209-273 (in place, unnormalised, batch contiguous N1*N2*N3 complex
  * float blocks, forward sign -1) and run on the GPU: each call copies the
- * batch to the device, transforms it in FP32 and copies it back.
+ * batch to the device, transforms it in FP32 and copies it back.  When every
+ * edge is a power of two the transform is fourn's (G.c:2756-2827) bit for
+ * bit, the result of the reference's -DUSE_FOURN build; edges with a factor
+ * 3 (which fourn does not take) get the FP32 Stockham passes.
  * pfdd_fft_host_init fixes the grid (the reference's N1..N3); it must be
  * called first and again to change the grid.  Returns 0 or a PFDD_E_* code;
  * the seam functions themselves return nothing, as in the reference, and

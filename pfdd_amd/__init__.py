@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG, "libpfdd_gpu.so")
 MAX_NS = 16
 OK, E_HIP, E_ALLOC, E_SHAPE, E_RADIX, E_NAN, E_UNSUPPORTED, E_STATE, E_LIMIT = 0, -1, -2, -3, -4, -5, -6, -7, -8
 NOALLOC_MAX_POINTS = 1 << 21
-FP64, MIXED, MIXED_LEAN = 0, 1, 2
+FP64, MIXED, MIXED_LEAN, REF32 = 0, 1, 2, 3
 B_STORED, B_RECOMPUTE = 0, 1
 FORWARD, BACKWARD = -1, 1
 PLAN_AUTO, PLAN_UNFUSED_ROWS, PLAN_UNFUSED_COLS, PLAN_FUSED_COLS, PLAN_NO_PLANE, PLAN_PLANE = 0, 1, 2, 4, 8, 16

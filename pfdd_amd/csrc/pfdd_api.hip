@@ -52,6 +52,7 @@ struct pfdd_ctx {
   int ntri = 0;
   int fp64 = 1;
   int lean = 0;              // PFDD_MIXED_LEAN: plain FP32 master
+  int ref32 = 0;             // PFDD_REF32: FP32 work array, FP64 master, the reference's float arithmetic
   int stored = 0;
   hipStream_t stream = nullptr;
   int owns_stream = 1;       // 0: the stream belongs to an ensemble
@@ -78,6 +79,7 @@ struct pfdd_ctx {
   int nline = 0;
   double* line_out = nullptr; // [HIST_CAP][F] line sums per step; row 0 also serves pfdd_line_sums
   Twiddles tw[3];
+  float2* fourn_tw[3][2] = {};  // PFDD_REF32: fourn's twiddles per axis, [0] forward, [1] backward
   StepArgs sa{};
   int upd_nblk = 0;
   int row_tile = 0;          // rows per block of the fused row kernel (0: not fused)
@@ -126,6 +128,12 @@ template <> const double2* tw_of<double>(const Twiddles& t) { return t.d; }
 template <> const float2* tw_of<float>(const Twiddles& t) { return t.f; }
 
 template <typename T> Master<T> xm(const pfdd_ctx* c) { return make_master<T>(c->xi, (long)c->F * c->P, c->lean != 0); }
+template <> Master<float> xm<float>(const pfdd_ctx* c) {
+  if (c->ref32) return Master<float>{nullptr, nullptr, (double2*)c->xi};
+  return make_master<float>(c->xi, (long)c->F * c->P, c->lean != 0);
+}
+// whether the master is the double2 array of the reference layout (FP64 and PFDD_REF32)
+bool master64(const pfdd_ctx* c) { return c->fp64 || c->ref32; }
 // Calls f with the context's master as the static form the per-step kernels
 // are instantiated for (Master<double>, MasterF32C or MasterF32L).
 template <typename T, typename Fn> void with_master(const pfdd_ctx* c, Fn&& f) {
@@ -243,8 +251,20 @@ int fft3d(pfdd_ctx* c, cx<T>* data, int batch, int sign) {
   return PFDD_OK;
 }
 
+// The contraction in the reference's float arithmetic (PFDD_REF32).
+int contract_r32(pfdd_ctx* c, int quirk) {
+  const dim3 grid((unsigned)((c->P + 255) / 256));
+  cx<float>* W = (cx<float>*)c->W;
+#define PFDD_CR(SM) hipLaunchKernelGGL((k_contract_r32<SM>), grid, dim3(256), 0, c->stream, W, c->ec, c->P, quirk)
+  if (c->S <= 4) PFDD_CR(4); else if (c->S <= 8) PFDD_CR(8); else if (c->S <= 12) PFDD_CR(12); else PFDD_CR(16);
+#undef PFDD_CR
+  tap(c, PFDD_STAGE_CONTRACT);
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+
 template <typename T>
 int contract(pfdd_ctx* c, int quirk) {
+  if (sizeof(T) == 4 && c->ref32) return contract_r32(c, quirk);
   constexpr int bs = contract_threads<T>();
   const dim3 grid((unsigned)((c->P + bs - 1) / bs), (unsigned)c->B);
   cx<T>* W = (cx<T>*)c->W;
@@ -593,6 +613,26 @@ int plane_launch(pfdd_ctx* c, bool finish = true, bool attr_only = false) {
 // One FFT pass of W along axis ax (0: N1, 1: N2, 2: N3) for the context's
 // layout: W is [outer][n][inner] with the fields in `outer` (field-major)
 // or innermost (interleaved).
+// One axis of fourn (PFDD_REF32) over lines of n points of an
+// [outer][n][inner] float array; the table is the axis's for `sign`.
+int fourn_axis(cx<float>* data, long outer, int n, long inner, const float2* tw, hipStream_t st) {
+  int logn = 0;
+  while ((1 << logn) < n) logn++;
+  const int nt = n / 2 >= FOURN_THREADS ? FOURN_THREADS : n / 2 > 64 ? n / 2 : 64;
+  hipLaunchKernelGGL(k_fourn_axis, dim3((unsigned)(outer * inner)), dim3(nt), 0, st, data, inner, n, logn, tw);
+  return hipPeekAtLastError() == hipSuccess ? PFDD_OK : PFDD_E_HIP;
+}
+const float2* fourn_tw_of(const pfdd_ctx* c, int ax, int sign) { return c->fourn_tw[ax][sign == PFDD_FORWARD ? 0 : 1]; }
+// fourn on batch contiguous N1*N2*N3 blocks: the axes N3, N2, N1 (G.c:2768-2826)
+int fourn3d(pfdd_ctx* c, float2* buf, int batch, int sign) {
+  const long n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3;
+  cx<float>* d = (cx<float>*)buf;
+  int rc;
+  if ((rc = fourn_axis(d, batch * n1 * n2, (int)n3, 1, fourn_tw_of(c, 2, sign), c->stream))) return rc;
+  if ((rc = fourn_axis(d, batch * n1, (int)n2, n3, fourn_tw_of(c, 1, sign), c->stream))) return rc;
+  return fourn_axis(d, batch, (int)n1, n2 * n3, fourn_tw_of(c, 0, sign), c->stream);
+}
+
 template <typename T>
 int pass(pfdd_ctx* c, int ax, int sign) {
   const long n1 = c->p.n1, n2 = c->p.n2, n3 = c->p.n3;
@@ -606,7 +646,11 @@ int pass(pfdd_ctx* c, int ax, int sign) {
   } else {
     outer = fo, n = (int)n1, inner = n2 * n3 * fi;
   }
-  const int rc = launch_axis<T>((cx<T>*)c->W, outer, n, inner, sign, tw_of<T>(c->tw[ax]), c->stream, rl_of(c, ax));
+  int rc;
+  if (sizeof(T) == 4 && c->ref32)
+    rc = fourn_axis((cx<float>*)c->W, outer, n, inner, fourn_tw_of(c, ax, sign), c->stream);
+  else
+    rc = launch_axis<T>((cx<T>*)c->W, outer, n, inner, sign, tw_of<T>(c->tw[ax]), c->stream, rl_of(c, ax));
   tap(c, ax == 0 ? PFDD_STAGE_FFT_AXIS1 : ax == 1 ? PFDD_STAGE_FFT_AXIS2 : PFDD_STAGE_FFT_AXIS3);
   return rc;
 }
@@ -677,6 +721,28 @@ int prologue(pfdd_ctx* c, int quirk) {
   return PFDD_OK;
 }
 
+// PFDD_REF32: the step and the prologue in the reference's order (G.c:4574-4644,
+// 4450-4452 + 1426-1447): fourn runs the axes N3, N2, N1 in both directions.
+int ref32_solve(pfdd_ctx* c, int quirk) {
+  int rc;
+  for (int ax = 2; ax >= 0; ax--)
+    if ((rc = pass<float>(c, ax, PFDD_FORWARD))) return rc;
+  if ((rc = contract<float>(c, quirk))) return rc;
+  for (int ax = 2; ax >= 0; ax--)
+    if ((rc = pass<float>(c, ax, PFDD_BACKWARD))) return rc;
+  return PFDD_OK;
+}
+int ref32_step(pfdd_ctx* c) {
+  const int rc = update<float>(c);
+  return rc ? rc : ref32_solve(c, 0);
+}
+int ref32_prologue(pfdd_ctx* c, int quirk) {
+  const long n = (long)c->F * c->P;
+  hipLaunchKernelGGL((k_xi_to_w<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<float>(c),
+                     (cx<float>*)c->W, n);
+  return ref32_solve(c, quirk);
+}
+
 template <typename T>
 void line_sums(pfdd_ctx* c, const int* step_ctr) {
   hipLaunchKernelGGL((k_line_sums<T>), dim3((unsigned)c->F), dim3(UPD_THREADS), 0, c->stream, xm<T>(c), c->P,
@@ -684,7 +750,7 @@ void line_sums(pfdd_ctx* c, const int* step_ctr) {
 }
 
 int step(pfdd_ctx* c) {
-  const int rc = c->fp64 ? one_step<double>(c) : one_step<float>(c);
+  const int rc = c->ref32 ? ref32_step(c) : c->fp64 ? one_step<double>(c) : one_step<float>(c);
   if (rc || c->nline == 0) return rc;
   if (c->fp64) line_sums<double>(c, c->step_ctr);
   else line_sums<float>(c, c->step_ctr);
@@ -708,6 +774,34 @@ int make_twiddles(Twiddles& t, int n, long long& bytes, hipStream_t st) {
   HIPCK(hipMemcpyAsync(t.f, hf.data(), sizeof(float2) * n, hipMemcpyHostToDevice, st));
   HIPCK(hipStreamSynchronize(st));
   bytes += (long long)n * 24;
+  return PFDD_OK;
+}
+
+// fourn's twiddles (G.c:2803-2823) for one axis of n = 2^m points and one
+// sign.  The stage of half-span h (ifp1/ip1 in fourn) uses theta = isign *
+// 6.28318530717959 / (2h) and the values (float)wr, (float)wi of the double
+// recurrence from wr = 1, wi = 0, one per j < h: stored at tw[h - 1 + j].
+// Evaluated in the reference's operation order without fused multiply-adds,
+// so the table holds the floats fourn multiplies by.
+int make_fourn_twiddles(float2*& out, int n, int isign, long long& bytes, hipStream_t st) {
+#pragma clang fp contract(off)
+  std::vector<float2> h((size_t)n - 1);
+  for (int half = 1; half < n; half <<= 1) {
+    const double theta = isign * 6.28318530717959 / (double)(2 * half);
+    const double wtemp = std::sin(0.5 * theta);
+    const double wpr = -2.0 * wtemp * wtemp, wpi = std::sin(theta);
+    double wr = 1.0, wi = 0.0;
+    for (int j = 0; j < half; j++) {
+      h[(size_t)half - 1 + j] = make_float2((float)wr, (float)wi);
+      const double wt = wr;
+      wr = wt * wpr - wi * wpi + wr;
+      wi = wi * wpr + wt * wpi + wi;
+    }
+  }
+  HIPCK(hipMalloc(&out, sizeof(float2) * h.size()));
+  HIPCK(hipMemcpyAsync(out, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice, st));
+  HIPCK(hipStreamSynchronize(st));
+  bytes += (long long)sizeof(float2) * (long long)h.size();
   return PFDD_OK;
 }
 
@@ -840,6 +934,9 @@ void pfdd_destroy(pfdd_ctx* c) {
     if (t.d) (void)hipFree(t.d);
     if (t.f) (void)hipFree(t.f);
   }
+  for (auto& ax : c->fourn_tw)
+    for (float2* t : ax)
+      if (t) (void)hipFree(t);
   if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -923,6 +1020,18 @@ int apply_plan(pfdd_ctx* c, int plan) {
   int row_tile = 0, row_nblk = 0, c3_tile = 0, plane = 0, il = 0;
   size_t c3_lds = 0;
   if ((plan & PFDD_PLAN_INTERLEAVED) && (plan & PFDD_PLAN_FIELD_MAJOR)) return PFDD_E_SHAPE;
+  if (c->ref32) {  // the unfused field-major kernels only: k_update, fourn passes, k_contract_r32
+    if (plan & ~(PFDD_PLAN_FIELD_MAJOR | PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_NO_PLANE))
+      return PFDD_E_SHAPE;
+    c->r4 = 0;
+    c->row_tile = c->row_nblk = c->c3_tile = 0;
+    c->c3_lds = 0;
+    c->plane = c->il = c->il_fixed = c->il_fixed_row = 0;
+    drop_graph(c);
+    c->plan = plan;
+    c->prologue_done = 0;
+    return PFDD_OK;
+  }
   // AUTO takes the interleaved layout where its compile-time row and column
   // kernels exist (S = 12 at N1 = N3 = 128 or 256, recomputed B) and
   // measured faster than the field-major plan: at 128^3 x 12 0.686 against
@@ -1009,12 +1118,17 @@ int apply_plan(pfdd_ctx* c, int plan) {
 
 int validate_params(const pfdd_params* p) {
   if (p->ns < 1 || p->ns > PFDD_MAX_NS) return PFDD_E_SHAPE;
-  if (p->precision != PFDD_FP64 && p->precision != PFDD_MIXED && p->precision != PFDD_MIXED_LEAN)
+  if (p->precision != PFDD_FP64 && p->precision != PFDD_MIXED && p->precision != PFDD_MIXED_LEAN &&
+      p->precision != PFDD_REF32)
     return PFDD_E_SHAPE;
   if (p->bmode != PFDD_B_STORED && p->bmode != PFDD_B_RECOMPUTE) return PFDD_E_SHAPE;
   const int ns3[3] = {p->n1, p->n2, p->n3};
   for (int i = 0; i < 3; i++)
-    if (!len_ok(ns3[i])) return PFDD_E_RADIX;
+    if (!len_ok(ns3[i]) || (p->precision == PFDD_REF32 && (ns3[i] & (ns3[i] - 1)))) return PFDD_E_RADIX;
+  // PFDD_REF32 runs the unfused field-major kernels only
+  if (p->precision == PFDD_REF32 &&
+      (p->plan & ~(PFDD_PLAN_FIELD_MAJOR | PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS | PFDD_PLAN_NO_PLANE)))
+    return PFDD_E_SHAPE;
   const double* chk[] = {&p->C11, &p->C12, &p->C44, &p->d1, &p->d2, &p->d3, &p->uq2, &p->pi_const};
   for (const double* v : chk)
     if (!std::isfinite(*v)) return PFDD_E_NAN;
@@ -1052,7 +1166,8 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
   c->ntri = c->S * (c->S + 1) / 2;
   c->fp64 = p->precision == PFDD_FP64;
   c->lean = p->precision == PFDD_MIXED_LEAN;
-  c->stored = p->bmode == PFDD_B_STORED;
+  c->ref32 = p->precision == PFDD_REF32;
+  c->stored = p->bmode == PFDD_B_STORED && !c->ref32;  // PFDD_REF32 forms B in FP64 at each k-point
   c->tauh.assign((size_t)c->F, make_double2(0.0, 0.0));
   c->inst_pp.assign((size_t)batch, 0);
   c->inst_frozen.assign((size_t)batch, 0);
@@ -1077,7 +1192,7 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     c->partials_len = (size_t)std::max<long>(std::max<long>((long)F * c->upd_nblk, 18 * strain_nblk),
                                              std::max<long>((long)F * p->n1 * p->n2, (long)F * p->n1)) +
                       (size_t)F * FOLD;
-    c->xi_bytes = (c->fp64 ? master_bytes<double>(false) : master_bytes<float>(c->lean != 0)) * F * P;
+    c->xi_bytes = (master64(c) ? master_bytes<double>(false) : master_bytes<float>(c->lean != 0)) * F * P;
     if (!(c->xi = dev_alloc<char>(c->xi_bytes, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->W = dev_alloc<char>(csz * F * P, c->bytes))) FAIL(PFDD_E_ALLOC);
     if (!(c->partials = (double*)dev_alloc<char>(sizeof(double) * c->partials_len, c->bytes))) FAIL(PFDD_E_ALLOC);
@@ -1095,6 +1210,11 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     for (int i = 0; i < 3; i++) {
       if ((rc = make_twiddles(c->tw[i], ns3[i], c->bytes, c->stream))) goto fail;
       if ((rc = cols_attr(ns3[i]))) goto fail;
+      if (c->ref32)
+        for (int d = 0; d < 2; d++)
+          if ((rc = make_fourn_twiddles(c->fourn_tw[i][d], ns3[i], d ? PFDD_BACKWARD : PFDD_FORWARD, c->bytes,
+                                        c->stream)))
+            goto fail;
     }
     // elastic constants (Bmatrix G.c:340-358; s_a = C:e_a etc.)
     ElasticConsts h{};
@@ -1166,6 +1286,8 @@ int ctx_create(const pfdd_params* p, int batch, hipStream_t shared, pfdd_ctx** o
     c->sa.nblk = c->upd_nblk;
     c->sa.uq2 = p->uq2;
     c->sa.pi = p->pi_const;
+    c->sa.uq2f = (float)p->uq2;
+    c->sa.nsizef = (float)P;
     for (int a = 0; a < S; a++) {
       c->sa.cd2[a] = p->cd2[a];
       c->sa.asf2[a] = p->asf2[a];
@@ -1265,7 +1387,7 @@ int upload_inst(pfdd_ctx* c, int b, const double* xi, const double* xi_bc, int n
     }
   }
   HIPCK(hipStreamSynchronize(c->stream));
-  if (c->fp64) {
+  if (master64(c)) {
     HIPCK(hipMemcpyAsync((double2*)c->xi + off, xi, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
   } else {  // split into the compensated form of Master<float> (or round, in the lean form)
     std::vector<float2> r(n);
@@ -1318,7 +1440,7 @@ int download_inst(pfdd_ctx* c, int b, double* xi, int nsv) {
   const long P = c->P;
   const size_t n = (size_t)c->S * P;
   HIPCK(hipStreamSynchronize(c->stream));
-  if (c->fp64) {
+  if (master64(c)) {
     HIPCK(hipMemcpyAsync(xi, (double2*)c->xi + (size_t)b * n, sizeof(double2) * n, hipMemcpyDeviceToHost,
                          c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
@@ -1346,7 +1468,9 @@ int download_inst(pfdd_ctx* c, int b, double* xi, int nsv) {
 
 int begin_outer(pfdd_ctx* c, int quirk) {
   if (!c->has_xi) return PFDD_E_STATE;
-  const int rc = c->fp64 ? prologue<double>(c, quirk ? 1 : 0) : prologue<float>(c, quirk ? 1 : 0);
+  const int rc = c->ref32  ? ref32_prologue(c, quirk ? 1 : 0)
+                 : c->fp64 ? prologue<double>(c, quirk ? 1 : 0)
+                           : prologue<float>(c, quirk ? 1 : 0);
   if (rc) return rc;
   c->prologue_done = 1;
   return PFDD_OK;
@@ -1483,6 +1607,7 @@ std::vector<int> plan_candidates(pfdd_ctx* c) {
 // the step counters are saved before and restored after each candidate, so
 // the context's state is as before except that a new prologue is needed.
 int autotune_ctx(pfdd_ctx* c, int steps, int* best_plan, float* best_ms) {
+  if (c->ref32) return PFDD_E_UNSUPPORTED;  // one plan only
   if (!c->has_xi) return PFDD_E_STATE;
   if (steps < 1) return PFDD_E_SHAPE;
   const size_t xbytes = c->xi_bytes;
@@ -1716,6 +1841,7 @@ int pfdd_fft(pfdd_ctx* c, void* dev_buf, int batch, int dir, int prec) {
   if (dir != PFDD_FORWARD && dir != PFDD_BACKWARD) return PFDD_E_SHAPE;
   if (prec == PFDD_FP64) return fft3d<double>(c, (double2*)dev_buf, batch, dir);
   if (prec == PFDD_MIXED) return fft3d<float>(c, (float2*)dev_buf, batch, dir);
+  if (prec == PFDD_REF32) return c->ref32 ? fourn3d(c, (float2*)dev_buf, batch, dir) : PFDD_E_STATE;
   return PFDD_E_SHAPE;
 }
 
@@ -1766,7 +1892,7 @@ int enqueue_stage(pfdd_ctx* c, int stage, int reps) {
         break;
       case PFDD_STAGE_STEP:
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->step_ctr, 0);
-        rc = one_step<T>(c);
+        rc = c->ref32 ? ref32_step(c) : one_step<T>(c);
         break;
       case PFDD_STAGE_COPY:
         hipLaunchKernelGGL((k_xi_to_w<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xm<T>(c), W, n);
@@ -2061,6 +2187,7 @@ int pfdd_create_ensemble(const pfdd_params* params, int count, pfdd_ens** out) {
   for (int i = 0; i < count; i++) {
     const int rc = validate_params(&params[i]);
     if (rc) return rc;
+    if (params[i].precision == PFDD_REF32) return PFDD_E_UNSUPPORTED;
   }
   pfdd_ens* e = new (std::nothrow) pfdd_ens();
   if (!e) return PFDD_E_ALLOC;
@@ -2328,6 +2455,12 @@ struct HostSeam {
   int dev = -1;    // device of ctx (current at pfdd_fft_host_init)
 };
 HostSeam g_seam;
+// The seam's transform: fourn bit for bit when every edge is a power of two
+// (the seam's context is then a PFDD_REF32 one), the FP32 Stockham passes
+// otherwise (edges with a factor 3, which fourn does not take).
+int seam_fft(pfdd_ctx* c, float2* buf, int batch, int dir) {
+  return c->ref32 ? fourn3d(c, buf, batch, dir) : fft3d<float>(c, buf, batch, dir);
+}
 
 // Staging buffer of the allocation-free seam call (pfdd_fft_host_noalloc):
 // a fixed device array, one per GPU for the life of the process.  In host
@@ -2352,7 +2485,7 @@ int seam_run(float* data, int batch, int dir) {
     g_seam.cap = n;
   }
   HIPCK(hipMemcpyAsync(g_seam.buf, data, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
-  const int rc = fft3d<float>(c, g_seam.buf, batch, dir);
+  const int rc = seam_fft(c, g_seam.buf, batch, dir);
   if (rc) return rc;
   HIPCK(hipMemcpyAsync(data, g_seam.buf, n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
@@ -2378,7 +2511,8 @@ int pfdd_fft_host_init(int n1, int n2, int n3) {
   p.n2 = n2;
   p.n3 = n3;
   p.ns = 1;
-  p.precision = PFDD_MIXED;
+  const auto pow2 = [](int v) { return v > 1 && (v & (v - 1)) == 0; };
+  p.precision = pow2(n1) && pow2(n2) && pow2(n3) ? PFDD_REF32 : PFDD_MIXED;
   p.bmode = PFDD_B_RECOMPUTE;
   p.C11 = 3.0;
   p.C12 = 1.0;
@@ -2412,7 +2546,7 @@ int pfdd_fft_host_noalloc(float* data, int batch, int dir) {
   // every element the transform reads is copied in first: nothing has to start at zero
   float2* buf = (float2*)g_scratch_ptr[g_seam.dev];
   HIPCK(hipMemcpyAsync(buf, data, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
-  const int rc = fft3d<float>(c, buf, batch, dir);
+  const int rc = seam_fft(c, buf, batch, dir);
   if (rc) return rc;
   HIPCK(hipMemcpyAsync(data, buf, n * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));

@@ -101,20 +101,28 @@ struct MasterF32L {  // lean: (Re xi, Im xi) as a plain FP32 complex
 };
 // Either form, chosen per access (the kernels that run once per outer
 // iteration or only in non-default plans).
+// A third form serves PFDD_REF32 (FP32 work array, FP64 master as in the
+// reference, G.c:4146): d points to the double2 master and r, im are null.
 template <> struct Master<float> {
   float2* r;  // (hi, lo) of Re xi; (Re xi, Im xi) in the lean form
   float* im;  // Im xi; null in the lean form
+  double2* d = nullptr;  // PFDD_REF32: the FP64 master
   using Raw = double2;
   __device__ __forceinline__ double2 load(long i) const {
+    if (d) return Master<double>{d}.load(i);
     return im ? MasterF32C{r, im}.load(i) : MasterF32L{r}.load(i);
   }
   __device__ __forceinline__ Raw raw(long i) const { return load(i); }
   static __device__ __forceinline__ double2 widen(Raw v) { return v; }
   __device__ __forceinline__ void store(long i, double2 v) const {
-    if (im) MasterF32C{r, im}.store(i, v);
+    if (d) Master<double>{d}.store(i, v);
+    else if (im) MasterF32C{r, im}.store(i, v);
     else MasterF32L{r}.store(i, v);
   }
-  __device__ __forceinline__ double re(long i) const { return im ? MasterF32C{r, im}.re(i) : MasterF32L{r}.re(i); }
+  __device__ __forceinline__ double re(long i) const {
+    if (d) return d[i].x;
+    return im ? MasterF32C{r, im}.re(i) : MasterF32L{r}.re(i);
+  }
 };
 // Bytes per point of the master, and the master over a base allocation of n
 // points.  The lean form (PFDD_MIXED_LEAN) keeps xi as a plain FP32 complex,
@@ -339,6 +347,7 @@ struct StepArgs {
   int ns;
   int nblk;             // blocks per field of k_update
   double uq2, pi;
+  float uq2f, nsizef;   // uq2 and nsize as the reference's float arithmetic sees them (PFDD_REF32)
   double cd2[MAXS], asf2[MAXS], dslip[MAXS];  // per slip system
   // per field f = instance * ns + system: (tau, rate); the s-th GL step
   // after the ramp was set uses tau + rate * s
@@ -386,6 +395,22 @@ __device__ __forceinline__ void gl_point(double2& x, double dre, double dim, con
   }
   x.x -= c.cd2 * (a.uq2 * dre * inv_n + c.asf2 * a.pi * sn - t / c.dsl);
   x.y -= a.uq2 * c.cd2 * (dim * inv_n);
+}
+// plasticevolv with the reference's own arithmetic on a float data2
+// (PFDD_REF32): uq2*DATA2/nsize and DATA2_im/nsize are float operations
+// (float uq2, float data2, int nsize converted to float), the rest is double,
+// evaluated left to right with every product and sum rounded on its own
+// (no fused multiply-adds), as the compiled reference does (G.c:3486-3489).
+template <bool TAUPP>
+__device__ __forceinline__ void gl_point_r32(double2& x, float dre, float dim, const FieldCoef& c, const StepArgs& a,
+                                             const double* __restrict__ taupp, long pt) {
+#pragma clang fp contract(off)
+  const double t = TAUPP ? taupp[c.tbase + pt * a.ns] + c.rate : c.tau;
+  const float qre = (a.uq2f * dre) / a.nsizef;
+  const float qim = dim / a.nsizef;
+  const double sn = sin(2.0 * a.pi * x.x);
+  x.x = x.x - c.cd2 * (((double)qre + c.asf2 * a.pi * sn) - t / c.dsl);
+  x.y = x.y - ((double)a.uq2f * c.cd2) * (double)qim;
 }
 template <typename M> constexpr bool lean_master() { return __is_same(M, MasterF32L); }
 
@@ -495,7 +520,8 @@ k_update(Master<T> xi, cx<T>* __restrict__ W, const uint8_t* __restrict__ mask,
       const cx<T> d = W[off + pt];
       if (!MASK || mask[off + pt] == 0) {
         if constexpr (sizeof(T) == 4) {
-          if (!xi.im) gl_point<TAUPP, true>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
+          if (xi.d) gl_point_r32<TAUPP>(x, d.x, d.y, c, a, taupp, pt);
+          else if (!xi.im) gl_point<TAUPP, true>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
           else gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
         } else {
           gl_point<TAUPP>(x, (double)d.x, (double)d.y, c, a, taupp, pt, inv_n);
@@ -795,6 +821,143 @@ k_contract(cx<T>* __restrict__ W, const T* __restrict__ Bsym, const ElasticConst
   // pass, which then measured faster with ordinary stores
   contract_point<T, SM, true, sizeof(T) == 8>(v, ec, pt, S, quirk, STORED, Bsym, P, IL ? W + pt * S : W + pt,
                                               IL ? 1 : P);
+}
+
+// ---------------------------------------------------------------- PFDD_REF32
+// The reference's float arithmetic, for checks against the compiled
+// reference itself: its FOURN transform bit for bit, its float data2
+// accumulation and its float uq2*data2/nsize.
+//
+// One axis of fourn (G.c:2758-2827) on lines of n = 2^logn points of an
+// [outer][n][inner] float array: the bit-reversal permutation, then the
+// radix-2 decimation-in-time stages, each butterfly in float with the
+// twiddle (float)wr, (float)wi of the reference's double recurrence
+// (tw: n - 1 entries, the half-span-h stage at tw[h - 1 + j]), every product
+// and sum rounded on its own.  The stages of one line are sequential and its
+// butterflies within a stage independent, so the order of work does not
+// change a bit.  fourn runs the axes N3, N2, N1 in both directions.
+// One block per line.
+constexpr int FOURN_THREADS = 256;
+__global__ void __launch_bounds__(FOURN_THREADS)
+k_fourn_axis(cx<float>* __restrict__ data, long inner, int n, int logn, const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
+  __shared__ cx<float> sm[1024];
+  const long line = blockIdx.x;
+  const long o = line / inner, i = line - o * inner;
+  cx<float>* base = data + o * n * inner + i;
+  for (int k = threadIdx.x; k < n; k += blockDim.x)
+    sm[k] = base[(long)(__brev((unsigned)k) >> (32 - logn)) * inner];
+  __syncthreads();
+  for (int h = 1; h < n; h <<= 1) {
+    for (int b = threadIdx.x; b < n / 2; b += blockDim.x) {
+      const int j = b & (h - 1), k1 = ((b - j) << 1) + j, k2 = k1 + h;
+      const float2 w = tw[h - 1 + j];
+      const cx<float> d1 = sm[k1], d2 = sm[k2];
+      const float tr = w.x * d2.x - w.y * d2.y;
+      const float ti = w.x * d2.y + w.y * d2.x;
+      sm[k2] = {d1.x - tr, d1.y - ti};
+      sm[k1] = {d1.x + tr, d1.y + ti};
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) base[(long)k * inner] = sm[k];
+}
+
+// Fmatrix's D_a (G.c:512-576) at one k-point, in Fmatrix's own arithmetic:
+// its G, A and D are float arrays (G.c:424-429), so every assignment to them
+// rounds to float, while the products and sums on the right are double.  The
+// loops keep Fmatrix's order (A_mnuv over i, j, k, l; D_a,ij over k, l), which
+// fixes the rounding; G_ki, which Fmatrix recomputes inside the loop nest with
+// the same inputs, is computed once.  C is the unrotated C_ijkl (Fmatrix's
+// C_rotate for the identity orientation).  D[a][i*3+j] for a < S.
+template <int SM>
+__device__ void fmatrix_d_r32(const double fk[3], const ElasticConsts* __restrict__ ec, int S, float (&D)[SM][9]) {
+#pragma clang fp contract(off)
+  for (int a = 0; a < SM; a++)
+    for (int t = 0; t < 9; t++) D[a][t] = 0.0f;
+  const double fk2 = fk[0] * fk[0] + fk[1] * fk[1] + fk[2] * fk[2];
+  if (!(fk2 > 0)) return;
+  const double mu = ec->mu, ll = ec->ll, mup = ec->mup;
+  double z2[3];
+  for (int i = 0; i < 3; i++) z2[i] = fk[i] * fk[i] / fk2;
+  const double kk =
+      1.0 + (mu + ll) * (1 / (mu + mup * z2[0]) * z2[0] + 1 / (mu + mup * z2[1]) * z2[1] + 1 / (mu + mup * z2[2]) * z2[2]);
+  float G[3][3];  // G[k][i]
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 3; i++)
+      G[k][i] = (float)(1 / fk2 *
+                        ((i == k ? 1 : 0) / (mu + mup * z2[k]) -
+                         fk[i] * fk[k] / fk2 / (mu + mup * z2[i]) / (mu + mup * z2[k]) * (mu + ll) / kk));
+  const double* C = ec->C;  // C[((i*3+j)*3+k)*3+l]
+  for (int mn = 0; mn < 9; mn++) {
+    float Arow[9];  // A[m][n][u][v]
+    for (int uv = 0; uv < 9; uv++) {
+      float A = 0.0f;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+              A = (float)((double)A - C[(k * 3 + l) * 9 + uv] * C[(i * 3 + j) * 9 + mn] * (double)G[k][i] * fk[j] * fk[l]);
+      Arow[uv] = (float)((double)A + C[mn * 9 + uv]);
+    }
+    for (int a = 0; a < SM; a++) {
+      if (a >= S) continue;
+      float d = 0.0f;
+      for (int kl = 0; kl < 9; kl++) d = (float)((double)d + (double)Arow[kl] * ec->eps[a][kl]);
+      D[a][mn] = d;
+    }
+  }
+}
+
+// The contraction in the reference's arithmetic (G.c:4609-4629): data2_a
+// starts at 0 and takes data_b * BB_ab for b ascending, each product and sum
+// in double and the sum rounded to float on every store.  B_ab is formed in
+// FP64 at the k-point and scaled by 1/mu (the reference's stored BB, G.c:395-401).  With
+// quirk, virtualevolv's nine terms data_a * DD (G.c:1426-1447) follow, in
+// the reference's order, with DD_a(v) = D_a[v % 3][v / 3] / mu (the slot its
+// index I5V(..., a, v) lands on, G.c:38, 56) and D_a from fmatrix_d_r32.
+template <int SM>
+__global__ void __launch_bounds__(256)
+k_contract_r32(cx<float>* __restrict__ W, const ElasticConsts* __restrict__ ec, long P, int quirk) {
+#pragma clang fp contract(off)
+  const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= P) return;
+  const int S = ec->ns;
+  cx<float> v[SM];
+#pragma unroll
+  for (int b = 0; b < SM; b++)
+    if (b < S) v[b] = W[(long)b * P + pt];
+  double f[3], g[6];
+  kvec(pt, ec, f);
+  const bool nz = green6(f, ec, g);
+  double u[SM][3];
+#pragma unroll
+  for (int b = 0; b < SM; b++)
+    if (b < S) sig_f(ec, b, f, u[b]);
+  float Dq[SM][9];  // Fmatrix's D (prologue only)
+  if (quirk) fmatrix_d_r32<SM>(f, ec, S, Dq);
+#pragma unroll
+  for (int a = 0; a < SM; a++) {
+    if (a >= S) continue;
+    double gu[3];
+    gmul(g, u[a], gu);
+    float ar = 0.0f, ai = 0.0f;
+#pragma unroll
+    for (int b = 0; b < SM; b++) {
+      if (b >= S) continue;
+      const double bab = nz ? (ec->SE[a][b] - (gu[0] * u[b][0] + gu[1] * u[b][1] + gu[2] * u[b][2])) * ec->inv_mu : 0.0;
+      ar = (float)((double)ar + (double)v[b].x * bab);
+      ai = (float)((double)ai + (double)v[b].y * bab);
+    }
+    if (quirk) {
+      for (int vv = 0; vv < 9; vv++) {
+        const double dd = (double)Dq[a][(vv % 3) * 3 + vv / 3] / ec->mu;
+        ar = (float)((double)ar + (double)v[a].x * dd);
+        ai = (float)((double)ai + (double)v[a].y * dd);
+      }
+    }
+    W[(long)a * P + pt] = {ar, ai};
+  }
 }
 
 // ---------------------------------------------------------------- fused kernels

@@ -21,7 +21,7 @@
  * presets: c0 (32^3 x 3, one loop, 500 steps), c1 (128^3 x 12, 16 loops,
  * 1000 steps), c2 (256^3 x 12, 64 loops, 2000 steps, mixed precision).
  * keys: n1 n2 n3 systems(plane111|fcc) nloops rmin rmax seed sigma_amp tau0
- *       steps nt converge plastic_max precision(fp64|mixed|mixed_lean) bmode(recompute|stored)
+ *       steps nt converge plastic_max precision(fp64|mixed|mixed_lean|ref32) bmode(recompute|stored)
  *       quirk pfout_every fields(1|0) energy(1|0) plan autotune restart(1|0) xi_in outdir
  *
  * Checkpoints (G.c:4361-4382, 4540-4547, 4777-4788): the run ends by writing
@@ -134,7 +134,10 @@ static void set_key(config *c, const char *kv)
   else if (!strcmp(key, "converge")) c->converge = atoi(v);
   else if (!strcmp(key, "plastic_max")) c->plastic_max = atoi(v);
   else if (!strcmp(key, "precision"))
-    c->precision = !strcmp(v, "mixed") ? PFDD_MIXED : !strcmp(v, "mixed_lean") ? PFDD_MIXED_LEAN : PFDD_FP64;
+    c->precision = !strcmp(v, "mixed")        ? PFDD_MIXED
+                   : !strcmp(v, "mixed_lean") ? PFDD_MIXED_LEAN
+                   : !strcmp(v, "ref32")      ? PFDD_REF32
+                                              : PFDD_FP64;
   else if (!strcmp(key, "bmode")) c->bmode = !strcmp(v, "stored") ? PFDD_B_STORED : PFDD_B_RECOMPUTE;
   else if (!strcmp(key, "quirk")) c->quirk = atoi(v);
   else if (!strcmp(key, "plan")) c->plan = (int)strtol(v, NULL, 0);

@@ -84,10 +84,15 @@ def test_create_validates_before_touching_the_device():
     assert _create_rc(p) == pa.E_SHAPE
     assert _create_rc(pa.make_params(**{**base, "C11": float("nan")})) == pa.E_NAN
     assert _create_rc(pa.make_params(**{**base, "dslip": [1.0, 0.0, 1.0]})) == pa.E_NAN
-    for bad in (7, 3, -1):
+    for bad in (7, 4, -1):
         p = pa.make_params(**base)
         p.precision = bad
         assert _create_rc(p) == pa.E_SHAPE
+    # PFDD_REF32: fourn's power-of-two edges and the unfused field-major kernels only
+    assert _create_rc(pa.make_params(**{**base, "shape": (32, 48, 32), "precision": pa.REF32})) == pa.E_RADIX
+    assert _create_rc(pa.make_params(**{**base, "shape": (32, 12, 32), "precision": pa.REF32})) == pa.E_RADIX
+    for plan in (pa.PLAN_INTERLEAVED, pa.PLAN_PLANE, pa.PLAN_FUSED_COLS, pa.plan_row_tile(2)):
+        assert _create_rc(pa.make_params(**{**base, "precision": pa.REF32, "plan": plan})) == pa.E_SHAPE
 
 
 @pytest.mark.parametrize("shape,pp", [((32, 2, 32), None), ((128, 128, 128), None), ((16, 12, 8), (3, 5, 6)),
