@@ -187,7 +187,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(w, budget_s=10.0):
+def cpu_baseline(w, budget_s=20.0):
     """The FP64 oracle on the same instance, on the cores this process may use
     (at most 16, the host share of one GPU) and on one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
