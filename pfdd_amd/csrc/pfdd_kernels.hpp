@@ -1377,13 +1377,17 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
   }
   __syncthreads();
   const int p = t / TPP, q = t % TPP;
-  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)-1, twl);
+  auto fft = [&](T sign) {
+    if constexpr (N == 128) fft128_lds_shfl<T, LD, SH>(sm, p, K::NPEN, q, sign, twl);
+    else fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, sign, twl);
+  };
+  fft((T)-1);
   for (int e = t; e < N * TC; e += NT) {
     const int c = e / N, n = e % N;  // consecutive threads read consecutive LDS words
     contract_stress_form_lds<T, S>(sm + c * LD + lds_pos<SH>(n), TC * LD, ec, (long)n * M + m0 + c);
   }
   __syncthreads();
-  fft_lds_multi<T, N, 1, LD, SH>(sm, p, 1 << 30, K::NPEN, q, (T)1, twl);
+  fft((T)1);
   {
     cx<T> buf[EL];
 #pragma unroll
