@@ -310,54 +310,62 @@ __device__ __forceinline__ void fft_lds_multi(cx<T>* sm, int p0, int pstride, in
 // positions (j / 8) * 64 + j % 8 + 8 r (r < 8), and the radix-2 pass pairs
 // position p < 64 with p + 64: those two sit in lanes j and j ^ 8.  One
 // swap of four values with the partner lane (__shfl_xor over 16 lanes)
-// gives each lane four complete pairs -- the lower lane the pairs
-// p = j + 8 r, r < 4, the upper one r >= 4 -- whose butterflies it finishes
-// and writes to LDS.  This drops the second pass's LDS stores, the third
-// pass's LDS loads and one barrier.  The first pass is the ordinary one.
-// Ends synchronised (the result is in LDS).
+// gives each lane four complete pairs -- the lower lane (j < 8) the pairs
+// p = j + 8 r for even r, the upper one for odd r -- whose butterflies it
+// finishes and writes to LDS.  This drops the second pass's LDS stores, the
+// third pass's LDS loads and one barrier.  The first pass is the ordinary
+// one.  Splitting the pairs by the parity of r makes the 16 lanes of a
+// pencil store 16 consecutive positions at a time.
+//
+// The twiddles come from a 120-entry table in the order the lanes read them
+// (Shfl128Tw), so that the 8 distinct factors a pass reads at once sit in 8
+// consecutive entries: entry (r - 1) * 8 + k = w^(2 k r) for the second
+// pass (k = j % 8, r = 1..7), entry 56 + m = w^m for the radix-2 pass
+// (m < 64), w = exp(+2 pi i / 128); the same factors as the plain table, so
+// the results are the same bits.  Ends synchronised (the result is in LDS).
 template <typename T, int LD, int SH>
 __device__ __forceinline__ void fft128_lds_shfl(cx<T>* sm, int p, int npen, int q, T s, const cx<T>* __restrict__ tw) {
   constexpr int N = 128;
   static_assert(make_plan(N).np == 3 && make_plan(N).r[0] == 8 && make_plan(N).r[1] == 8 &&
                     make_plan(N).r[2] == 2 && make_plan(N).tpp == 16, "plan 8, 8, 2 with 16 lanes");
   const LdsPencils<T, SH> l{sm, LD};
-  stockham_pass<T, N, 8, 1, 16, 1>(l, l, p, 1 << 30, npen, q, s, tw);  // ends with a barrier
+  stockham_pass<T, N, 8, 1, 16, 1>(l, l, p, 1 << 30, npen, q, s, tw);  // no twiddles; ends with a barrier
   cx<T> v[8];
   const bool on = p < npen;
+  const int j8 = q % 8;
   if (on) {
-    // second pass: NSB = 8, M = 16 butterflies (one per lane), twiddles tw[(j % 8) * r * 2]
+    // second pass: NSB = 8, M = 16 butterflies (one per lane), twiddles w^(2 (j % 8) r)
 #pragma unroll
     for (int r = 0; r < 8; r++) v[r] = l.load(p, q + r * 16);
-    const int k = q % 8;
 #pragma unroll
     for (int r = 1; r < 8; r++) {
-      cx<T> w = tw[k * r * 2];
+      cx<T> w = tw[(r - 1) * 8 + j8];
       w.y *= s;
       v[r] = cmul<T>(v[r], w);
     }
     Bfly<T, 8>::run(v, s);
   }
-  // exchange with lane j ^ 8: the lower lane (j < 8) keeps v[0..3] and takes
-  // the partner's v[0..3] (positions 64 + j + 8 r); the upper lane keeps
-  // v[4..7] and takes the partner's v[4..7] (positions j % 8 + 8 r)
+  // exchange with lane j ^ 8: the lower lane keeps v[2i] (position j + 16 i)
+  // and takes the partner's v[2i] (position 64 + j + 16 i); the upper lane
+  // keeps v[2i + 1] (position 64 + j % 8 + 8 (2i + 1)) and takes the
+  // partner's v[2i + 1]
   const bool lower = q < 8;
-  const int j8 = q % 8;
   cx<T> a[4], b[4];  // pair (position pp, pp + 64): a at pp, b at pp + 64
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const cx<T> send = lower ? v[4 + i] : v[i];
+    const cx<T> send = lower ? v[2 * i + 1] : v[2 * i];
     cx<T> got;
     got.x = __shfl_xor(send.x, 8, 16);
     got.y = __shfl_xor(send.y, 8, 16);
-    a[i] = lower ? v[i] : got;
-    b[i] = lower ? got : v[4 + i];
+    a[i] = lower ? v[2 * i] : got;
+    b[i] = lower ? got : v[2 * i + 1];
   }
   __syncthreads();  // every lane's second-pass inputs are read before the stores below
   if (on) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int pp = j8 + 8 * (lower ? i : 4 + i);  // third pass: NSB = 64, twiddle tw[pp]
-      cx<T> w = tw[pp];
+      const int pp = j8 + 8 * (2 * i + (lower ? 0 : 1));  // third pass: NSB = 64, twiddle w^pp
+      cx<T> w = tw[56 + pp];
       w.y *= s;
       cx<T> pr[2] = {a[i], cmul<T>(b[i], w)};
       Bfly<T, 2>::run(pr, s);
@@ -368,7 +376,29 @@ __device__ __forceinline__ void fft128_lds_shfl(cx<T>* sm, int p, int npen, int 
   __syncthreads();
 }
 
-// One pencil per TPP-thread group: pencil p = threadIdx.x / TPP at sm + p*LD.// One pencil per TPP-thread group: pencil p = threadIdx.x / TPP at sm + p*LD.
+// The twiddle table of fft128_lds_shfl, staged into LDS like TwStage: entry
+// e < 56 is tw[2 (e % 8) (e / 8 + 1)], entry 56 + m is tw[m], from the
+// plain 128-entry table tw[m] = w^m.
+template <typename T, int NT> struct Shfl128Tw {
+  static constexpr int E = 120, K = (E + NT - 1) / NT;
+  cx<T> v[K];
+  __device__ __forceinline__ void load(const cx<T>* __restrict__ tw) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int e = (int)threadIdx.x + k * NT;
+      if (e < E) v[k] = tw[e < 56 ? 2 * (e % 8) * (e / 8 + 1) : e - 56];
+    }
+  }
+  __device__ __forceinline__ void store(cx<T>* twl) const {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int e = (int)threadIdx.x + k * NT;
+      if (e < E) twl[e] = v[k];
+    }
+  }
+};
+
+// One pencil per TPP-thread group: pencil p = threadIdx.x / TPP at sm + p*LD.
 template <typename T, int N, int LD>
 __device__ __forceinline__ void fft_lds(cx<T>* sm, int npen, T s, const cx<T>* __restrict__ tw) {
   constexpr int TPP = make_plan(N).tpp;

@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "pfdd_fft.hpp"
 
@@ -1334,7 +1335,11 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
 // arithmetic per k-point and per pencil as k_col_contract_il.
 // grid = (column tiles, instances).
 template <typename T, int N, int S, int TC> struct ColIlFixed {
-  static constexpr int TPP = make_plan(N).tpp, SH = 3, LD = padded_pitch<N, SH>(), NPEN = S * TC, ROW = S * TC;
+  // One pad slot per 2^SH positions.  At N = 128 (the shuffle transform) one
+  // per 16: the 16 lanes of a pencil then read and write 16 consecutive
+  // positions on distinct LDS banks in every pass and in the contraction.
+  static constexpr int TPP = make_plan(N).tpp, SH = N == 128 ? 4 : 3, LD = padded_pitch<N, SH>(), NPEN = S * TC,
+                       ROW = S * TC;
   // Minimum waves per SIMD asked of the register allocator: the FP32 form
   // fits in the 72 registers that seven waves allow without spilling (at
   // N = 128 four blocks per CU instead of three: 0.124 -> 0.102 ms at
@@ -1369,7 +1374,8 @@ k_col_il_fixed(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, long
     cx<T> buf[EL];
 #pragma unroll
     for (int k = 0; k < EL; k++) buf[k] = gsrc[(unsigned)(k * KN * (int)rs)];
-    TwStage<T, N, NT> ts;  // (the twiddle loads after the tile's, stored with it)
+    // (the twiddle loads after the tile's, stored with it)
+    std::conditional_t<N == 128, Shfl128Tw<T, NT>, TwStage<T, N, NT>> ts;
     ts.load(tw);
 #pragma unroll
     for (int k = 0; k < EL; k++) lcol[k * (KN + (KN >> SH))] = buf[k];
