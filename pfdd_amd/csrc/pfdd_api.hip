@@ -390,19 +390,22 @@ int k1il_launch_n(pfdd_ctx* c, bool attr_only) {
   return rc;
 }
 
-// Lengths with compile-time interleaved row and column kernels (S = 12).
-constexpr bool il_fixed_len(int n) { return n == 128 || n == 256; }
+// The number of fields the compile-time interleaved row and column kernels
+// are instantiated for at each length (0: none): S = 12 at 128 and 256 (the
+// 12 FCC systems), S = 3 at 512 (three systems on a 512^3 grid).
+constexpr int il_fixed_S(int n) { return n == 128 || n == 256 ? 12 : n == 512 ? 3 : 0; }
 // Rows per block of the compile-time interleaved row kernel (k_row_il_fixed)
 // for ns fields along n3, or 0 where it is not instantiated.
-int il_fixed_rows(int S, int n3) { return S == 12 && il_fixed_len(n3) ? 1 : 0; }
+int il_fixed_rows(int S, int n3) { return S == il_fixed_S(n3) ? 1 : 0; }
 template <typename T, int N>
 bool k1il_fixed(pfdd_ctx* c) {
-  if constexpr (il_fixed_len(N)) {
-    if (!c->il_fixed_row || c->mask || c->taupp) return false;
+  if constexpr (il_fixed_S(N) > 0) {
+    constexpr int SF = il_fixed_S(N);
+    if (!c->il_fixed_row || c->S != SF || c->mask || c->taupp) return false;
     with_master<T>(c, [&](auto xi) {
       using M = decltype(xi);
-      hipLaunchKernelGGL((k_row_il_fixed<T, N, 12, 1, M>), dim3((unsigned)c->row_nblk, (unsigned)c->B),
-                         dim3(RowIlFixed<T, N, 12, 1>::NT), 0, c->stream, (cx<T>*)c->W, xi, c->sa,
+      hipLaunchKernelGGL((k_row_il_fixed<T, N, SF, 1, M>), dim3((unsigned)c->row_nblk, (unsigned)c->B),
+                         dim3(RowIlFixed<T, N, SF, 1>::NT), 0, c->stream, (cx<T>*)c->W, xi, c->sa,
                          tw_of<T>(c->tw[2]), c->partials, c->step_ctr);
     });
     return true;
@@ -490,22 +493,22 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
 // Tile of the compile-time interleaved column kernel (k_col_il_fixed) for
 // ns fields along n1, or 0 where it is not instantiated.
 int il_fixed_tile(int S, int n1, bool fp64) {
-  if (S != 12 || !il_fixed_len(n1)) return 0;
+  if (S != il_fixed_S(n1)) return 0;
   return fp64 ? 1 : 2;
 }
 template <typename T, int N>
 void k3il_fixed(pfdd_ctx* c) {
-  constexpr int TC = sizeof(T) == 8 ? 1 : 2;
+  constexpr int TC = sizeof(T) == 8 ? 1 : 2, SF = il_fixed_S(N);
   const long nb = ((long)c->p.n2 * c->p.n3) / TC;
-  hipLaunchKernelGGL((k_col_il_fixed<T, N, 12, TC>), dim3((unsigned)nb, (unsigned)c->B),
-                     dim3(ColIlFixed<T, N, 12, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P,
+  hipLaunchKernelGGL((k_col_il_fixed<T, N, SF, TC>), dim3((unsigned)nb, (unsigned)c->B),
+                     dim3(ColIlFixed<T, N, SF, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P,
                      tw_of<T>(c->tw[0]));
 }
 
 // Column kernel, interleaved layout (c->c3_tile columns of every field per block).
 template <typename T, int N>
 int k3il_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
-  if constexpr (il_fixed_len(N)) {
+  if constexpr (il_fixed_S(N) > 0) {
     if (!attr_only && c->il_fixed && !quirk) {  // (the prologue's virtualevolv term takes the generic kernel)
       k3il_fixed<T, N>(c);
       tap(c, PFDD_STAGE_CONTRACT);
@@ -1033,13 +1036,17 @@ int apply_plan(pfdd_ctx* c, int plan) {
     return PFDD_OK;
   }
   // AUTO takes the interleaved layout where its compile-time row and column
-  // kernels exist (S = 12 at N1 = N3 = 128 or 256, recomputed B) and
-  // measured faster than the field-major plan: at 128^3 x 12 0.686 against
-  // 0.909 ms per FP64 step and 0.400 against 0.466 ms per mixed step; at
-  // 256^3 x 12 3.82 against 4.52 ms (mixed) and 3.43 against 3.80 ms (lean
-  // mixed), but 8.60 against 7.54 ms in FP64, where the row kernel's 60 KB
-  // of LDS leave two blocks per CU, so FP64 at 256 stays field-major.
-  // Without the compile-time kernels the field-major plans measured faster.
+  // kernels exist (S = 12 at N1 = N3 = 128 or 256, S = 3 at 512; recomputed
+  // B) and measured faster than the field-major plan: at 128^3 x 12 0.686
+  // against 0.909 ms per FP64 step and 0.400 against 0.466 ms per mixed
+  // step; at 256^3 x 12 3.82 against 4.52 ms (mixed) and 3.43 against 3.80
+  // ms (lean mixed), but 8.60 against 7.54 ms in FP64, where the row
+  // kernel's 60 KB of LDS leave two blocks per CU; at 512^3 x 3 9.24
+  // against 9.47 ms (mixed) and 8.60 against 8.84 ms (lean mixed), but
+  // 17.2 against 16.9 ms in FP64, where the column kernel's 48-byte row
+  // segments (three fields of one column) run at 1.8 TB/s.  So FP64 takes
+  // it at 128 only.  Without the compile-time kernels the field-major plans
+  // measured faster.
   const bool auto_il = !(plan & (PFDD_PLAN_FIELD_MAJOR | PFDD_PLAN_UNFUSED_ROWS | PFDD_PLAN_UNFUSED_COLS |
                                  PFDD_PLAN_FUSED_COLS | PFDD_PLAN_PLANE | PFDD_PLAN_NO_PLANE)) &&
                        !c->stored && il_fixed_tile(S, p.n1, c->fp64 != 0) > 0 && il_fixed_rows(S, p.n3) > 0 &&

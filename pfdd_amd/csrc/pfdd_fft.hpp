@@ -385,8 +385,8 @@ template <typename T, int NT> struct Shfl128Tw {
   __device__ __forceinline__ void load(const cx<T>* __restrict__ tw) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const int e = (int)threadIdx.x + k * NT;
-      if (e < E) v[k] = tw[e < 56 ? 2 * (e % 8) * (e / 8 + 1) : e - 56];
+      const int e = (int)threadIdx.x + k * NT < E ? (int)threadIdx.x + k * NT : E - 1;  // (as in TwStage)
+      v[k] = tw[e < 56 ? 2 * (e % 8) * (e / 8 + 1) : e - 56];
     }
   }
   __device__ __forceinline__ void store(cx<T>* twl) const {
@@ -428,7 +428,9 @@ template <typename T, int N, int NT> struct TwStage {
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int i = (int)threadIdx.x + k * NT;
-      if (N % NT == 0 || i < N) v[k] = tw[i];
+      // every thread loads (an entry in range): loads under a branch were
+      // kept in scratch memory, each waited for before the next was issued
+      v[k] = tw[N % NT == 0 || i < N ? i : N - 1];
     }
   }
   __device__ __forceinline__ void store(cx<T>* twl) const {

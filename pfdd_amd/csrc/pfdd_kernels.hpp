@@ -1335,11 +1335,18 @@ k_col_contract_il(cx<T>* __restrict__ W, const ElasticConsts* __restrict__ ec, l
 // arithmetic per k-point and per pencil as k_col_contract_il.
 // grid = (column tiles, instances).
 template <typename T, int N, int S, int TC> struct ColIlFixed {
-  // One pad slot per 2^SH positions.  At N = 128 (the shuffle transform) one
-  // per 16: the 16 lanes of a pencil then read and write 16 consecutive
-  // positions on distinct LDS banks in every pass and in the contraction.
-  static constexpr int TPP = make_plan(N).tpp, SH = N == 128 ? 4 : 3, LD = padded_pitch<N, SH>(), NPEN = S * TC,
-                       ROW = S * TC;
+  // Pencil pitch LD and one pad slot per 2^SH positions, chosen by the LDS
+  // bank rules measured on gfx950 (64 banks of 4 bytes; a 16-byte read
+  // serves 16 lanes a cycle, in the lane groups {0-3, 12-15, 20-27} and
+  // {4-11, 16-19, 28-31} of each half-wave; a 16-byte write serves 8
+  // consecutive lanes a cycle across 32 banks; an 8-byte read serves 32
+  // lanes).  At N = 128 the FP64 kernel takes one pad per 8 points and pitch
+  // 147 (the fewest conflict cycles of the pitches and paddings that fit),
+  // the FP32 kernel one pad per 16 points (its pencils' 16 lanes then read
+  // and write 16 consecutive positions on distinct banks); elsewhere one pad
+  // per 8 points.
+  static constexpr int TPP = make_plan(N).tpp, SH = N == 128 && sizeof(T) == 4 ? 4 : 3,
+                       LD = padded_pitch<N, SH>() + (N == 128 && sizeof(T) == 8 ? 2 : 0), NPEN = S * TC, ROW = S * TC;
   // Minimum waves per SIMD asked of the register allocator: the FP32 form
   // fits in the 72 registers that seven waves allow without spilling (at
   // N = 128 four blocks per CU instead of three: 0.124 -> 0.102 ms at

@@ -14,6 +14,10 @@ from pfdd_amd import hip
 
 pytestmark = pytest.mark.gpu
 IL = pa.PLAN_INTERLEAVED
+# whether AUTO takes the compile-time interleaved kernels, per (N1 = N3, S) and precision
+AUTO_IL = {(128, 12): {pa.FP64: True, pa.MIXED: True, pa.MIXED_LEAN: True},
+           (256, 12): {pa.FP64: False, pa.MIXED: True, pa.MIXED_LEAN: True},
+           (512, 3): {pa.FP64: False, pa.MIXED: True, pa.MIXED_LEAN: True}}
 
 
 def say(msg):
@@ -140,23 +144,24 @@ def test_interleaved_ensemble_equals_single(prec):
         assert np.array_equal(g[:, i], g1)
 
 
-@pytest.mark.parametrize("n", [128, 256])
+@pytest.mark.parametrize("n,ns", [(128, 12), (256, 12), (512, 3)])
 @pytest.mark.parametrize("prec", [pa.FP64, pa.MIXED, pa.MIXED_LEAN])
-def test_fixed_kernels_equal_field_major(n, prec):
-    """At S = 12 with N1 = N3 in {128, 256} and recomputed B the interleaved
-    plan runs the compile-time row and column kernels (k_row_il_fixed,
-    k_col_il_fixed).  They do the field-major kernels' arithmetic per pencil
-    and per k-point, so xi comes out bit for bit the same; gamma only
-    regroups its block sums.  AUTO takes them except in FP64 at 256."""
-    say(f"fixed interleaved kernels n={n} prec={prec}")
-    w = C.small_fcc((n, 8, n), ns=12, nloops=8, rmin=6.0, rmax=20.0)
+def test_fixed_kernels_equal_field_major(n, ns, prec):
+    """With recomputed B the interleaved plan runs the compile-time row and
+    column kernels (k_row_il_fixed, k_col_il_fixed) at S = 12 with N1 = N3
+    in {128, 256} and at S = 3 with N1 = N3 = 512.  They do the field-major
+    kernels' arithmetic per pencil and per k-point, so xi comes out bit for
+    bit the same; gamma only regroups its block sums.  AUTO takes them except
+    in FP64 at 256."""
+    say(f"fixed interleaved kernels n={n} ns={ns} prec={prec}")
+    w = C.small_fcc((n, 8, n), ns=ns, nloops=8, rmin=6.0, rmax=20.0)
     x0, g0, _, _, _ = run_gpu(w, 16, pa.PLAN_FIELD_MAJOR | pa.PLAN_NO_PLANE, precision=prec)
     x1, g1, _, sp, _ = run_gpu(w, 16, IL, precision=prec)
     assert sp[pa.STAGE_FFT_AXIS3] == 0 and sp[pa.STAGE_FFT_AXIS1] == 0   # fused rows and fused columns
     assert np.array_equal(x1, x0)
     assert np.abs(g1 - g0).max() <= 1e-14 * max(1.0, np.abs(g0).max())
     x2, _, _, sp2, _ = run_gpu(w, 16, 0, precision=prec)
-    auto_il = not (prec == pa.FP64 and n == 256)
+    auto_il = AUTO_IL[(n, ns)][prec]
     assert sp2[pa.STAGE_FFT_AXIS1] == (0 if auto_il else 2)
     assert np.array_equal(x2, x0)
     if prec == pa.FP64:
