@@ -492,13 +492,21 @@ int k3_launch_n(pfdd_ctx* c, int quirk, bool attr_only) {
 
 // Tile of the compile-time interleaved column kernel (k_col_il_fixed) for
 // ns fields along n1, or 0 where it is not instantiated.
+// Columns per block: one in FP64; two in FP32, four at S = 3 (N = 512),
+// where a column's row segment is only 3 fields long: 96-byte segments
+// instead of 48 took the FP32 kernel from 3.9 to 3.1 ms at 512^3 x 3 (two
+// columns in FP64, 63 KB of LDS and two blocks per CU, went from 7.1 to
+// 8.2 ms).
+template <typename T, int N> constexpr int il_fixed_tc() {
+  return sizeof(T) == 8 ? 1 : N == 512 ? 4 : 2;
+}
 int il_fixed_tile(int S, int n1, bool fp64) {
   if (S != il_fixed_S(n1)) return 0;
-  return fp64 ? 1 : 2;
+  return fp64 ? 1 : n1 == 512 ? 4 : 2;
 }
 template <typename T, int N>
 void k3il_fixed(pfdd_ctx* c) {
-  constexpr int TC = sizeof(T) == 8 ? 1 : 2, SF = il_fixed_S(N);
+  constexpr int TC = il_fixed_tc<T, N>(), SF = il_fixed_S(N);
   const long nb = ((long)c->p.n2 * c->p.n3) / TC;
   hipLaunchKernelGGL((k_col_il_fixed<T, N, SF, TC>), dim3((unsigned)nb, (unsigned)c->B),
                      dim3(ColIlFixed<T, N, SF, TC>::NT), 0, c->stream, (cx<T>*)c->W, c->ec, c->P,
@@ -986,7 +994,7 @@ int il_tiles(pfdd_ctx* c, int plan, int& row_tile, int& c3_tile, size_t& c3_lds)
     if (creq) {
       t = 1 << (creq - 1);
       if (!col_fits(t)) return PFDD_E_SHAPE;
-    } else if (const int ft = il_fixed_tile(S, p.n1, c->fp64 != 0); ft > 0) {
+    } else if (const int ft = il_fixed_tile(S, p.n1, c->fp64 != 0); ft > 0 && inner % ft == 0) {
       t = ft;
     } else {
       while (col_fits(t * 2) && col_lds(t * 2) <= IL_AUTO_LDS && (size_t)t * 2 * S * csz <= 256) t *= 2;
@@ -1041,8 +1049,8 @@ int apply_plan(pfdd_ctx* c, int plan) {
   // against 0.909 ms per FP64 step and 0.400 against 0.466 ms per mixed
   // step; at 256^3 x 12 3.82 against 4.52 ms (mixed) and 3.43 against 3.80
   // ms (lean mixed), but 8.60 against 7.54 ms in FP64, where the row
-  // kernel's 60 KB of LDS leave two blocks per CU; at 512^3 x 3 9.24
-  // against 9.47 ms (mixed) and 8.60 against 8.84 ms (lean mixed), but
+  // kernel's 60 KB of LDS leave two blocks per CU; at 512^3 x 3 8.34
+  // against 9.47 ms (mixed) and 7.71 against 8.84 ms (lean mixed), but
   // 17.2 against 16.9 ms in FP64, where the column kernel's 48-byte row
   // segments (three fields of one column) run at 1.8 TB/s.  So FP64 takes
   // it at 128 only.  Without the compile-time kernels the field-major plans
