@@ -1510,7 +1510,12 @@ k_plane_step(cx<T>* __restrict__ W, M xi, const uint8_t* __restrict__ mask,
              const double* __restrict__ taupp, StepArgs a, const cx<T>* __restrict__ tw2,
              const cx<T>* __restrict__ tw3, double* __restrict__ partials, const int* __restrict__ step_ctr) {
   extern __shared__ unsigned char lds_raw[];
-  plane_body<T, N2, N3, MASK, TAUPP, M>(W, xi, mask, taupp, a, tw2, tw3, partials, step_ctr, blockIdx.x,
+  // The two axes' twiddle tables in LDS, read there by the passes (stored
+  // before the first pass, whose barrier orders them before the second).
+  __shared__ cx<T> twl2[N2], twl3[N3];
+  stage_twiddles<T, N2>(twl2, tw2);
+  stage_twiddles<T, N3>(twl3, tw3);
+  plane_body<T, N2, N3, MASK, TAUPP, M>(W, xi, mask, taupp, a, twl2, twl3, partials, step_ctr, blockIdx.x,
                                      reinterpret_cast<cx<T>*>(lds_raw));
 }
 
